@@ -1,0 +1,201 @@
+// frg_device.h -- device primitives for the CDNA4 (gfx950) path tracker.
+//
+//  * cx<R>       complex number over R = double or float ({re, im}, bit-compatible with the
+//                reference's Cplx for R = double, classes.cuh:26-33).
+//  * group ops   8-lane groups inside a wave64 (one path per group): butterfly sums, maxima,
+//                arg-max and broadcasts built on DPP (quad_perm and row_half_mirror), so every
+//                lane of a group ends with bitwise identical results.
+//  * Philox4x32-10 (Salmon et al., SC'11) counter-based RNG.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace frg {
+
+// ------------------------------------------------------------------------------------------
+// complex arithmetic
+// ------------------------------------------------------------------------------------------
+template <typename R>
+struct cx {
+    R re, im;
+};
+
+template <typename R>
+__device__ __forceinline__ cx<R> mk(R a, R b) {
+    cx<R> z;
+    z.re = a;
+    z.im = b;
+    return z;
+}
+template <typename R>
+__device__ __forceinline__ cx<R> operator+(cx<R> a, cx<R> b) { return mk(a.re + b.re, a.im + b.im); }
+template <typename R>
+__device__ __forceinline__ cx<R> operator-(cx<R> a, cx<R> b) { return mk(a.re - b.re, a.im - b.im); }
+template <typename R>
+__device__ __forceinline__ cx<R> operator-(cx<R> a) { return mk(-a.re, -a.im); }
+template <typename R>
+__device__ __forceinline__ cx<R> operator*(cx<R> a, cx<R> b) {
+    return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <typename R>
+__device__ __forceinline__ cx<R> operator*(R s, cx<R> a) { return mk(s * a.re, s * a.im); }
+template <typename R>
+__device__ __forceinline__ R abs2(cx<R> a) { return a.re * a.re + a.im * a.im; }
+template <typename R>
+__device__ __forceinline__ cx<R> conj(cx<R> a) { return mk(a.re, -a.im); }
+// 1/a = conj(a)/|a|^2 (one real reciprocal); values here stay far from over/underflow.
+template <typename R>
+__device__ __forceinline__ cx<R> inv(cx<R> a) {
+    R r = R(1) / abs2(a);
+    return mk(a.re * r, -a.im * r);
+}
+template <typename R>
+__device__ __forceinline__ cx<R> cdiv(cx<R> a, cx<R> b) { return a * inv(b); }
+// acc - a*b
+template <typename R>
+__device__ __forceinline__ cx<R> fms(cx<R> acc, cx<R> a, cx<R> b) {
+    return mk(acc.re - a.re * b.re + a.im * b.im, acc.im - a.re * b.im - a.im * b.re);
+}
+// acc + a*b
+template <typename R>
+__device__ __forceinline__ cx<R> fma(cx<R> acc, cx<R> a, cx<R> b) {
+    return mk(acc.re + a.re * b.re - a.im * b.im, acc.im + a.re * b.im + a.im * b.re);
+}
+template <typename R, typename S>
+__device__ __forceinline__ cx<R> cvt(cx<S> a) { return mk(R(a.re), R(a.im)); }
+template <typename R>
+__device__ __forceinline__ cx<R> sel(bool c, cx<R> a, cx<R> b) { return mk(c ? a.re : b.re, c ? a.im : b.im); }
+
+// Principal square root by the half-angle formulas of classes.cuh:39-45.
+__device__ __forceinline__ cx<double> csqrt_ref(cx<double> x) {
+    double r = sqrt(x.re * x.re + x.im * x.im);
+    double cosA = x.re / r;
+    cx<double> o = mk(sqrt(r * (cosA + 1.0) / 2.0), sqrt(r * (1.0 - cosA) / 2.0));
+    if (signbit(x.im)) o.im = -o.im;
+    return o;
+}
+
+// ------------------------------------------------------------------------------------------
+// 8-lane group primitives (DPP).  DPP controls (gfx9): quad_perm [1,0,3,2] = 0xB1,
+// quad_perm [2,3,0,1] = 0x4E, row_half_mirror = 0x141 (lane i <-> 7-i within each 8 lanes).
+// ------------------------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double x) {
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    lo = dpp_i<CTRL>(lo);
+    hi = dpp_i<CTRL>(hi);
+    return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x) {
+    return __int_as_float(dpp_i<CTRL>(__float_as_int(x)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double x) { return dpp_d<CTRL>(x); }
+template <int CTRL>
+__device__ __forceinline__ float dpp(float x) { return dpp_f<CTRL>(x); }
+
+template <typename R>
+__device__ __forceinline__ R grp_sum(R x) {
+    x = x + dpp<0xB1>(x);
+    x = x + dpp<0x4E>(x);
+    x = x + dpp<0x141>(x);
+    return x;
+}
+template <typename R>
+__device__ __forceinline__ R grp_max(R x) {
+    x = fmax(x, dpp<0xB1>(x));
+    x = fmax(x, dpp<0x4E>(x));
+    x = fmax(x, dpp<0x141>(x));
+    return x;
+}
+template <typename R>
+__device__ __forceinline__ cx<R> grp_sum(cx<R> z) { return mk(grp_sum(z.re), grp_sum(z.im)); }
+
+// Arg-max over the group: larger key wins, ties go to the smaller id.  Keys must not be NaN.
+template <int CTRL, typename R>
+__device__ __forceinline__ void argmax_step(R &key, int &id) {
+    R k2 = dpp<CTRL>(key);
+    int i2 = dpp_i<CTRL>(id);
+    bool take = (k2 > key) || (k2 == key && i2 < id);
+    key = take ? k2 : key;
+    id = take ? i2 : id;
+}
+template <typename R>
+__device__ __forceinline__ int grp_argmax(R key, int id) {
+    argmax_step<0xB1>(key, id);
+    argmax_step<0x4E>(key, id);
+    argmax_step<0x141>(key, id);
+    return id;
+}
+
+// Value of lane `src` (0..7) of the group, delivered to every lane (sum with zeros elsewhere).
+template <typename R>
+__device__ __forceinline__ R grp_bcast(R x, int role, int src) { return grp_sum(role == src ? x : R(0)); }
+template <typename R>
+__device__ __forceinline__ cx<R> grp_bcast(cx<R> z, int role, int src) {
+    return mk(grp_bcast(z.re, role, src), grp_bcast(z.im, role, src));
+}
+__device__ __forceinline__ int grp_bcast_i(int v, int role, int src) {
+    int x = (role == src) ? v : 0;
+    x = x + dpp_i<0xB1>(x);
+    x = x + dpp_i<0x4E>(x);
+    x = x + dpp_i<0x141>(x);
+    return x;
+}
+
+// Orders this lane's LDS accesses before and after the call at compiler level.  LDS requests of
+// one wave are processed in issue order, so a write by one lane is seen by a later read of
+// another lane of the same wave.
+__device__ __forceinline__ void lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ------------------------------------------------------------------------------------------
+// Philox4x32-10.  Draw k of a path is half (k & 1) of block k >> 1; a half is the 53 high bits
+// of a 64-bit word plus one, times 2^-53, giving a double in (0, 1].
+// ------------------------------------------------------------------------------------------
+struct u4 {
+    uint32_t x, y, z, w;
+};
+__device__ __forceinline__ u4 philox(uint64_t blk, uint64_t pid, uint64_t seed) {
+    uint32_t c0 = (uint32_t)blk, c1 = (uint32_t)(blk >> 32), c2 = (uint32_t)pid, c3 = (uint32_t)(pid >> 32);
+    uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    u4 o;
+    o.x = c0;
+    o.y = c1;
+    o.z = c2;
+    o.w = c3;
+    return o;
+}
+__device__ __forceinline__ double u53(uint32_t lo, uint32_t hi) {
+    uint64_t w = ((uint64_t)hi << 32) | lo;
+    return (double)((w >> 11) + 1) * 0x1.0p-53;
+}
+// The two uniforms of block `blk`: draws 2*blk (first) and 2*blk+1 (second).
+__device__ __forceinline__ void uniform_pair(uint64_t blk, uint64_t pid, uint64_t seed, double &a, double &b) {
+    u4 o = philox(blk, pid, seed);
+    a = u53(o.x, o.y);
+    b = u53(o.z, o.w);
+}
+
+}  // namespace frg

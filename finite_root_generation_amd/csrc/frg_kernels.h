@@ -1,0 +1,52 @@
+// frg_kernels.h -- kernel arguments shared by the tracker kernels and the host C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "frg_device.h"
+
+namespace frg {
+
+// Tracker settings (defaults: htpytrack.cu:9-31, frgkinematics.cu:7-8).
+struct TrackOpts {
+    double sinit, smax, smin, rk_tol;
+    int caution_steps, newt_its;
+    double newt_tol;
+    int newt_attempts, sharp_its;
+    double sharp_tol;
+    int sharp_attempts, pad0;
+    double sharp_trigger, nudge_trigger;
+    int max_steps, max_ministeps;
+    double patch_lo, patch_hi, start_lo, start_hi;
+    int start_mode;     // 0: U[start_lo, start_hi]^2 (reference), 1: standard complex Gaussian
+    int precision;      // 0: FP64, 1: FP32 Jacobian/solve away from t = 1
+    double fp64_band;   // mixed precision: FP64 once 1 - t < fp64_band
+};
+
+// One launch.  Several instances (task targets) may share a launch; instance i owns global path
+// indices [inst_first[i], inst_first[i+1]).
+struct Job {
+    int64_t n_total;
+    int n_inst;
+    const int64_t *inst_first;    // n_inst + 1 (may be null when n_inst == 1)
+    const cx<double> *fpm;        // n_inst x 30 target parameters
+    const cx<double> *gamma;      // n_inst
+    const uint64_t *seed;         // n_inst RNG seeds
+    const uint64_t *pid0;         // n_inst: RNG path id of the instance's first path
+    const cx<double> *spt, *spm;  // explicit start systems (n_total x 21, n_total x 30) or null
+    cx<double> *fpt;              // n_total x 21 output roots (AoS, the reference's Vect<21>)
+    int *err;                     // n_total error codes 0..5
+    int *stats;                   // n_total x 8 counters or null
+    cx<double> *yfinal;           // n_total x 23 homogeneous end points or null
+    unsigned long long *queue;    // work-queue head (persistent launches)
+    TrackOpts o;
+};
+
+// host-side launchers (frg_tracker.hip)
+enum LaunchKind : int { LK_STATIC = 0, LK_PERSISTENT = 1 };
+hipError_t launch_track(const Job &job, int kind, int block, int grid, hipStream_t stream);
+size_t track_lds_bytes(int block, int precision);
+int track_max_groups_per_cu(int block, int kind, int precision);
+
+}  // namespace frg

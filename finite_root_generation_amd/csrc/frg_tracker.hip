@@ -1,0 +1,967 @@
+// frg_tracker.hip -- the MI355X (gfx950) homotopy path tracker.
+//
+// Replaces the reference's Homotopy kernel (frg.cu:35-69) and everything it inlines:
+// GenerateStart (frgkinematics.cu:13-136), TrackPath (htpytrack.cu:33-220), DormandPrince65
+// (rungekutta.cu:3-124), IVP/Newton/LinearSolve* (htpysupport.cu:146-494), FSYS/JVAR/JPAR
+// (htpykinematics.cu) and CognateSolns (frgkinematics.cu:227-285).
+//
+// Mapping.  One path per 8-lane group, eight paths per wave64.  Lane r < 7 owns pose j = r+1:
+// it evaluates equations I_j, II_j, III_j and their Jacobian rows.  Lane 7 owns the two patch
+// rows (projective patch u and velocity/time row w).  A 23-vector Y = [z(21), Z0, t] is held
+// as three complex slots per lane: lane r < 7 holds z[2r], z[2r+1], U_{r+1} (= z[14+r]); lane 7
+// holds Z0, t and a zero.
+//
+// Linear solves.  Every tangent (IVP) and Newton step solves the 23x23 system
+//     [ JH (rows 0..21, row-scaled) ; w ] x = b
+// (htpysupport.cu:250-283, :431-477).  U_j appears only in the three rows of pose j and in the
+// two patch rows, so the solve is done in three stages instead of one dense LU:
+//   1. in-lane: pivot among the three rows of pose j on |dF/dU_j|, eliminate U_j, leaving two
+//      rows over the 16 shared columns (14 linkage, Z0, t) and one pivot row;
+//   2. eliminate U_1..U_7 from the two patch rows (columns split over the 8 lanes via LDS);
+//   3. Gauss-Jordan with partial pivoting on the remaining 16x16 system, two rows per lane,
+//      pivot search by DPP arg-max, pivot row broadcast through LDS; then back-substitute U_j.
+// The Newton system (LinearSolveNewton, htpysupport.cu:302-400) is the same solve with w = e_t
+// and right-hand side dt = tgoal - t, which pins x_t exactly as the reference does.
+//
+// Control flow.  Each group runs TrackPath as a state machine whose every step is one solve, so
+// all groups of a wave execute the expensive part together whatever phase their path is in;
+// only the cheap bookkeeping diverges.  A persistent variant refills a group from a device work
+// queue when its path ends.
+#include "frg_device.h"
+#include "frg_system.h"
+#include "frg_kernels.h"
+
+namespace frg {
+
+// ------------------------------------------------------------------------------------------
+// RK6(5)8M tableau (rungekutta.cu:43-122).  kA[i][m]: stage i point = Yc + s * sum_m kA[i][m] k_m.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ double rk_a(int i, int m) {
+    // i, m are small runtime ints; resolved by selects (no memory traffic)
+    double v = 0.0;
+    v = (i == 1 && m == 0) ? 1. / 10. : v;
+    v = (i == 2 && m == 0) ? -2. / 81. : v;
+    v = (i == 2 && m == 1) ? 20. / 81. : v;
+    v = (i == 3 && m == 0) ? 615. / 1372. : v;
+    v = (i == 3 && m == 1) ? -270. / 343. : v;
+    v = (i == 3 && m == 2) ? 1053. / 1372. : v;
+    v = (i == 4 && m == 0) ? 3243. / 5500. : v;
+    v = (i == 4 && m == 1) ? -54. / 55. : v;
+    v = (i == 4 && m == 2) ? 50949. / 71500. : v;
+    v = (i == 4 && m == 3) ? 4998. / 17875. : v;
+    v = (i == 5 && m == 0) ? -26492. / 37125. : v;
+    v = (i == 5 && m == 1) ? 72. / 55. : v;
+    v = (i == 5 && m == 2) ? 2808. / 23375. : v;
+    v = (i == 5 && m == 3) ? -24206. / 37125. : v;
+    v = (i == 5 && m == 4) ? 338. / 459. : v;
+    v = (i == 6 && m == 0) ? 5561. / 2376. : v;
+    v = (i == 6 && m == 1) ? -35. / 11. : v;
+    v = (i == 6 && m == 2) ? -24117. / 31603. : v;
+    v = (i == 6 && m == 3) ? 899983. / 200772. : v;
+    v = (i == 6 && m == 4) ? -5225. / 1836. : v;
+    v = (i == 6 && m == 5) ? 3925. / 4056. : v;
+    v = (i == 7 && m == 0) ? 465467. / 266112. : v;
+    v = (i == 7 && m == 1) ? -2945. / 1232. : v;
+    v = (i == 7 && m == 2) ? -5610201. / 14158144. : v;
+    v = (i == 7 && m == 3) ? 10513573. / 3212352. : v;
+    v = (i == 7 && m == 4) ? -424325. / 205632. : v;
+    v = (i == 7 && m == 5) ? 376225. / 454272. : v;
+    return v;
+}
+// error (rungekutta.cu:99-105) and propagated solution (:115-122) weights; k2 enters neither
+__constant__ double kE[8] = {-13. / 2400., 0., 19683. / 618800., -2401. / 31200., 65. / 816., -15. / 416.,
+                              -521. / 5600., 1. / 10.};
+__constant__ double kB[8] = {61. / 864., 0., 98415. / 321776., 16807. / 146016., 1375. / 7344., 1375. / 5408.,
+                              -37. / 1120., 1. / 10.};
+
+// ------------------------------------------------------------------------------------------
+// Per-group LDS scratch
+// ------------------------------------------------------------------------------------------
+template <typename R>
+struct GroupLds {
+    cx<R> phat[7][17];  // normalised pivot row of each pose (16 shared columns + rhs)
+    cx<R> uw[14];       // U-entries of the (scaled) u row [0..6] and of the w row [7..13]
+    cx<R> a[32];        // eval point (16) / reduced patch rows (32) / pivot row (17) / solution (16)
+};
+
+struct Slots {
+    cx<double> v[3];
+};
+
+// Y-vector index held in slot k of lane `role` (-1: the unused third slot of lane 7).
+__device__ __forceinline__ int slot_index(int role, int k) {
+    if (role < 7) return k == 2 ? 14 + role : 2 * role + k;
+    return k == 2 ? -1 : 21 + k;
+}
+
+// ------------------------------------------------------------------------------------------
+// The group solve.  Inputs are this lane's slots of the evaluation point Ye, the patch u and the
+// last row w; `newton` selects the Newton right-hand side (-H scaled; w rhs = dt) over the
+// tangent one (zeros; w rhs = 1).  fnewt (FP64 residual values of this lane's three
+// equations) is used when `newton`.  Output: this lane's slots of x.
+// ------------------------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ void group_solve(int role, GroupLds<R> *L, const cx<double> Ye[3], const cx<double> uu[3],
+                                            const cx<double> ww[3], bool newton, double w_rhs,
+                                            const cx<double> s6[6], const cx<double> d6[6], cx<double> gamma,
+                                            const cx<double> fnewt[3], cx<double> x[3]) {
+    constexpr bool kOwnF = sizeof(R) == sizeof(double);  // FP64 rows carry F themselves
+    const cx<R> zero = mk(R(0), R(0));
+    const int pr = role < 7 ? role : 6;  // LDS row index used by this lane for pose data
+
+    // -------- patch-row scale and residual (group reductions, FP64) --------
+    double um = 0.0;
+    cx<double> uy = mk(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        um = fmax(um, fmax(fabs(uu[k].re), fabs(uu[k].im)));
+        uy = fma(uy, uu[k], Ye[k]);
+    }
+    um = grp_max(um);
+    uy = grp_sum(uy);
+    const double usc = 1.0 / um;
+    const cx<double> bu = newton ? mk(-(uy.re - 1.0) * usc, -uy.im * usc) : mk(0.0, 0.0);
+
+    // -------- 1. broadcast the shared part of the evaluation point --------
+    L->a[2 * role] = cvt<R>(Ye[0]);
+    L->a[2 * role + 1] = cvt<R>(Ye[1]);
+    lds_sync();
+    PoseEval<R> e;
+#pragma unroll
+    for (int k = 0; k < 14; ++k) e.L[k] = L->a[k];
+    e.Z0 = L->a[14];
+    e.t = L->a[15].re;
+    lds_sync();
+    e.U = role < 7 ? cvt<R>(Ye[2]) : mk(R(1), R(0));
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+        e.s[p] = cvt<R>(s6[p]);
+        e.d[p] = cvt<R>(d6[p]);
+    }
+    e.gamma = cvt<R>(gamma);
+
+    // -------- 2. pose rows, row scaling, in-lane elimination of U_j --------
+    Row<R> r0, r1, r2;
+    pose_rows(e, r0, r1, r2);
+    cx<R> rhs0 = zero, rhs1 = zero, rhs2 = zero;
+    {
+        R m0 = fmax(fabs(r0.au.re), fabs(r0.au.im)), m1 = fmax(fabs(r1.au.re), fabs(r1.au.im)),
+          m2 = fmax(fabs(r2.au.re), fabs(r2.au.im));
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            m0 = fmax(m0, fmax(fabs(r0.g[k].re), fabs(r0.g[k].im)));
+            m1 = fmax(m1, fmax(fabs(r1.g[k].re), fabs(r1.g[k].im)));
+            m2 = fmax(m2, fmax(fabs(r2.g[k].re), fabs(r2.g[k].im)));
+        }
+        const R s0 = R(1) / m0, s1 = R(1) / m1, s2 = R(1) / m2;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            r0.g[k] = s0 * r0.g[k];
+            r1.g[k] = s1 * r1.g[k];
+            r2.g[k] = s2 * r2.g[k];
+        }
+        r0.au = s0 * r0.au;
+        r1.au = s1 * r1.au;
+        r2.au = s2 * r2.au;
+        if (newton) {
+            const cx<double> f0 = kOwnF ? cvt<double>(r0.f) : fnewt[0];
+            const cx<double> f1 = kOwnF ? cvt<double>(r1.f) : fnewt[1];
+            const cx<double> f2 = kOwnF ? cvt<double>(r2.f) : fnewt[2];
+            rhs0 = cvt<R>(mk(-f0.re * double(s0), -f0.im * double(s0)));
+            rhs1 = cvt<R>(mk(-f1.re * double(s1), -f1.im * double(s1)));
+            rhs2 = cvt<R>(mk(-f2.re * double(s2), -f2.im * double(s2)));
+        }
+    }
+    // pivot among the three rows (first maximum wins, as in LinearSolve)
+    const R k0 = abs2(r0.au), k1 = abs2(r1.au), k2 = abs2(r2.au);
+    int pi = (k1 > k0) ? 1 : 0;
+    const R kb = (k1 > k0) ? k1 : k0;
+    pi = (k2 > kb) ? 2 : pi;
+    const bool p0 = pi == 0, p1 = pi == 1;
+    cx<R> pg[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) pg[k] = sel(p0, r0.g[k], sel(p1, r1.g[k], r2.g[k]));
+    const cx<R> pau = sel(p0, r0.au, sel(p1, r1.au, r2.au));
+    const cx<R> prhs = sel(p0, rhs0, sel(p1, rhs1, rhs2));
+    const cx<R> ipau = inv(pau);
+    // the two non-pivot rows, in order
+    cx<R> ra[17], rb[17];
+    {
+        const cx<R> ma = sel(p0, r1.au, r0.au) * ipau;  // row a = first non-pivot
+        const cx<R> mb = sel(pi == 2, r1.au, r2.au) * ipau;  // row b = second non-pivot
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            ra[k] = fms(sel(p0, r1.g[k], r0.g[k]), ma, pg[k]);
+            rb[k] = fms(sel(pi == 2, r1.g[k], r2.g[k]), mb, pg[k]);
+        }
+        ra[16] = fms(sel(p0, rhs1, rhs0), ma, prhs);
+        rb[16] = fms(sel(pi == 2, rhs1, rhs2), mb, prhs);
+    }
+    // normalised pivot row -> LDS; U-entries of u (scaled) and w -> LDS
+    const cx<R> uU = cvt<R>(usc * uu[2]);
+    const cx<R> wU = cvt<R>(ww[2]);
+    const cx<R> rho = prhs * ipau;
+    if (role < 7) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) L->phat[pr][k] = pg[k] * ipau;
+        L->phat[pr][16] = rho;
+        L->uw[pr] = uU;
+        L->uw[7 + pr] = wU;
+    }
+    lds_sync();
+
+    // -------- 3. eliminate U_1..U_7 from the patch rows; lane r builds columns 2r, 2r+1 --------
+    {
+        cx<R> cu0 = cvt<R>(usc * uu[0]), cu1 = cvt<R>(usc * uu[1]);
+        cx<R> cw0 = cvt<R>(ww[0]), cw1 = cvt<R>(ww[1]);
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            const cx<R> a0 = L->phat[j][2 * role], a1 = L->phat[j][2 * role + 1];
+            const cx<R> mu = L->uw[j], mw = L->uw[7 + j];
+            cu0 = fms(cu0, mu, a0);
+            cu1 = fms(cu1, mu, a1);
+            cw0 = fms(cw0, mw, a0);
+            cw1 = fms(cw1, mw, a1);
+        }
+        lds_sync();
+        L->a[2 * role] = cu0;
+        L->a[2 * role + 1] = cu1;
+        L->a[16 + 2 * role] = cw0;
+        L->a[16 + 2 * role + 1] = cw1;
+    }
+    // right-hand sides of the reduced patch rows: b - sum_j (U-entry_j * rho_j)
+    const cx<R> su = grp_sum(role < 7 ? uU * rho : zero);
+    const cx<R> sw = grp_sum(role < 7 ? wU * rho : zero);
+    lds_sync();
+    if (role == 7) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            ra[k] = L->a[k];
+            rb[k] = L->a[16 + k];
+        }
+        ra[16] = cvt<R>(bu) - su;
+        rb[16] = mk(R(w_rhs), R(0)) - sw;
+    }
+    lds_sync();
+
+    // -------- 4. Gauss-Jordan on the 16 x 16 system (rows 2*role, 2*role+1) --------
+    bool usedA = false, usedB = false;
+    int pcA = 0, pcB = 0;
+    cx<R> invA = zero, invB = zero;
+    const int idA = 2 * role, idB = 2 * role + 1;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+        R ka = abs2(ra[c]), kbv = abs2(rb[c]);
+        ka = (ka == ka) ? ka : R(-1);  // NaN keys lose to every number
+        kbv = (kbv == kbv) ? kbv : R(-1);
+        ka = usedA ? R(-2) : ka;
+        kbv = usedB ? R(-2) : kbv;
+        const bool bBetter = kbv > ka;
+        const int gid = grp_argmax(bBetter ? kbv : ka, bBetter ? idB : idA);
+        const bool pA = gid == idA, pB = gid == idB;
+        if (pA) {
+            usedA = true;
+            pcA = c;
+            invA = inv(ra[c]);
+        }
+        if (pB) {
+            usedB = true;
+            pcB = c;
+            invB = inv(rb[c]);
+        }
+        if (pA || pB) {
+            L->a[c] = pA ? invA : invB;
+#pragma unroll
+            for (int k = 0; k < 17; ++k)
+                if (k > c) L->a[k] = pA ? ra[k] : rb[k];
+        }
+        lds_sync();
+        const cx<R> ip = L->a[c];
+        const cx<R> la = pA ? zero : ra[c] * ip;
+        const cx<R> lb = pB ? zero : rb[c] * ip;
+#pragma unroll
+        for (int k = 0; k < 17; ++k) {
+            if (k > c) {
+                const cx<R> pk = L->a[k];
+                ra[k] = fms(ra[k], la, pk);
+                rb[k] = fms(rb[k], lb, pk);
+            }
+        }
+        lds_sync();
+    }
+    // -------- 5. solution of the shared unknowns --------
+    L->a[pcA] = ra[16] * invA;
+    L->a[pcB] = rb[16] * invB;
+    lds_sync();
+    x[0] = cvt<double>(L->a[2 * role]);
+    x[1] = cvt<double>(L->a[2 * role + 1]);
+    // -------- 6. back-substitute U_j = rho - phat . x --------
+    {
+        cx<R> acc = L->phat[pr][16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc = fms(acc, L->phat[pr][k], L->a[k]);
+        x[2] = role < 7 ? cvt<double>(acc) : mk(0.0, 0.0);
+    }
+    lds_sync();
+}
+
+// ------------------------------------------------------------------------------------------
+// Tracker state
+// ------------------------------------------------------------------------------------------
+enum Phase : int { PH_INIT = 0, PH_RK = 1, PH_NEWTON = 2, PH_SHARP = 3 };
+
+struct PathState {
+    // vectors (this lane's slots)
+    cx<double> Yc[3], Vc[3], u[3], K[8][3];  // K[0] doubles as YN, K[1] as Yn after an RK attempt
+    // pose parameters of this lane (start values, target - start)
+    cx<double> s6[6], d6[6], f6[6];
+    cx<double> gamma;
+    // scalars (identical in all lanes of the group)
+    double s, err, scale, residual, tgoal;
+    int phase, stage, code, attempt, its, step, ministep, streak;
+    bool tstep, done;
+    uint64_t draw, pid, seed;
+    int64_t path;
+    int inst;
+    int st_rk, st_newton_calls, st_newton_its, st_redraws, st_ivp;
+};
+
+// Patch draw (htpytrack.cu:55-58,147-150): Y index i uses draws base + 2i, base + 2i + 1.
+__device__ __forceinline__ void draw_vec(int role, const PathState &P, uint64_t base, double lo, double hi,
+                                         cx<double> out[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int yi = slot_index(role, k);
+        double a, b;
+        uniform_pair(base / 2 + (uint64_t)(yi < 0 ? 0 : yi), P.pid, P.seed, a, b);
+        out[k] = yi < 0 ? mk(0.0, 0.0) : mk((hi - lo) * a + lo, (hi - lo) * b + lo);
+    }
+}
+
+// SwitchPatch (htpysupport.cu:9-34): Y' = (1 - u_t t)/(sum_{i<22} u_i Y_i) * Y, t unchanged.
+__device__ __forceinline__ void switch_patch(int role, const cx<double> u[3], cx<double> Y[3]) {
+    cx<double> den = mk(0.0, 0.0), num = mk(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const bool is_t = (role == 7 && k == 1);
+        den = is_t ? den : fma(den, u[k], Y[k]);
+        num = is_t ? u[k] * Y[k] : num;
+    }
+    den = grp_sum(den);
+    num = grp_sum(num);
+    const cx<double> sc = cdiv(mk(1.0 - num.re, -num.im), den);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const bool is_t = (role == 7 && k == 1);
+        Y[k] = is_t ? Y[k] : sc * Y[k];
+    }
+}
+
+__device__ __forceinline__ double grp_norm(const cx<double> v[3]) {
+    double a = abs2(v[0]) + abs2(v[1]) + abs2(v[2]);
+    return sqrt(grp_sum(a));
+}
+// t (real part of Y[22], held by lane 7 slot 1), delivered to the whole group
+__device__ __forceinline__ double grp_t(int role, const cx<double> Y[3]) { return grp_bcast(Y[1].re, role, 7); }
+
+// ------------------------------------------------------------------------------------------
+// Start system: GenerateStart (frgkinematics.cu:13-136) for pose j = role+1 (lane 7 repeats
+// pose 7 and keeps only Z0 = 1, t = 0).
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ cx<double> start_value(const TrackOpts &o, uint64_t blk, const PathState &P) {
+    double a, b;
+    uniform_pair(blk, P.pid, P.seed, a, b);
+    if (o.start_mode == 1) {
+        const double r = sqrt(-log(a)), th = 6.283185307179586 * b;
+        double sn, cs;
+        sincos(th, &sn, &cs);
+        return mk(r * cs, r * sn);
+    }
+    return mk((o.start_hi - o.start_lo) * a + o.start_lo, (o.start_hi - o.start_lo) * b + o.start_lo);
+}
+
+__device__ __forceinline__ void generate_start(int role, const TrackOpts &o, PathState &P) {
+    const int j = role < 7 ? role + 1 : 7;
+    const cx<double> one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
+    cx<double> z[14];
+#pragma unroll
+    for (int i = 0; i < 14; ++i) z[i] = start_value(o, (uint64_t)i, P);
+    const cx<double> P0 = start_value(o, 14, P), Pc0 = start_value(o, 15, P);
+    const cx<double> R = start_value(o, 15 + (uint64_t)j, P), iR = inv(R);
+    double pmA, pmB;
+    uniform_pair(22 + (uint64_t)j, P.pid, P.seed, pmA, pmB);
+    const cx<double> A = z[0], B = z[1], C = z[2], D = z[3], F = z[4], G = z[5], H = z[6];
+    const cx<double> Ac = z[7], Bc = z[8], Cc = z[9], Dc = z[10], Fc = z[11], Gc = z[12], Hc = z[13];
+    cx<double> Aq = (G - D) * (iR * (Hc - Gc) - (Hc - Fc));
+    cx<double> Acq = (Gc - Dc) * (R * (H - G) - (H - F));
+    cx<double> Bq = ((zero - R * ((H - G) * (Hc - Fc))) - iR * ((Hc - Gc) * (H - F))) +
+                    ((G - D) * (Gc - Dc) + ((H - G) * (Hc - Gc) + ((H - F) * (Hc - Fc) - (F - D) * (Fc - Dc))));
+    double pm = pmA < 0.5 ? -1.0 : 1.0;
+    const cx<double> TUc = inv(mk(2.0, 0.0) * Aq) *
+                           ((zero - Bq) + mk(pm, 0.0) * csqrt_ref(Bq * Bq - mk(4.0, 0.0) * (Aq * Acq)));
+    const cx<double> SUc = inv(F - D) * (TUc * (G - D) + (R * (H - G) - (H - F)));
+    const cx<double> iSUc = inv(SUc);
+    const cx<double> E = (R * (H - C) - SUc * (F - B)) - (H - F);
+    const cx<double> Ec = (iR * (Hc - Cc) - iSUc * (Fc - Bc)) - (Hc - Fc);
+    Aq = (Ac - Bc) * E;
+    Acq = (A - B) * Ec;
+    Bq = E * Ec + ((A - B) * (Ac - Bc) - (C - A) * (Cc - Ac));
+    pm = pmB < 0.5 ? -1.0 : 1.0;
+    const cx<double> U = inv(mk(2.0, 0.0) * Aq) *
+                         ((zero - Bq) + mk(pm, 0.0) * csqrt_ref(Bq * Bq - mk(4.0, 0.0) * (Aq * Acq)));
+    const cx<double> QUc = mk(-1.0, 0.0) * inv(C - A) * (inv(U) * (A - B) + E);
+    const cx<double> Q = QUc * U, Qc = inv(Q);
+    const cx<double> S = SUc * U;
+    const cx<double> Pj = B + (S * (F - B) + U * (P0 - F));
+    const cx<double> Pcj = Bc + (inv(S) * (Fc - Bc) + inv(U) * (Pc0 - Fc));
+    P.s6[0] = P0;
+    P.s6[1] = Pj;
+    P.s6[2] = Pc0;
+    P.s6[3] = Pcj;
+    P.s6[4] = Q;
+    P.s6[5] = Qc;
+    if (role < 7) {
+        cx<double> a0 = z[0], a1 = z[1];
+#pragma unroll
+        for (int i = 1; i < 7; ++i) {
+            a0 = sel(role == i, z[2 * i], a0);
+            a1 = sel(role == i, z[2 * i + 1], a1);
+        }
+        P.Yc[0] = a0;
+        P.Yc[1] = a1;
+        P.Yc[2] = U;
+    } else {
+        P.Yc[0] = one;
+        P.Yc[1] = zero;
+        P.Yc[2] = zero;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Path set-up (Homotopy kernel start + TrackPath initialisation, htpytrack.cu:45-77)
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, int64_t path) {
+    const TrackOpts &o = J.o;
+    P.path = path;
+    int inst = 0;
+    if (J.n_inst > 1) {
+        int lo = 0, hi = J.n_inst - 1;  // last instance whose first path <= path
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (J.inst_first[mid] <= path) lo = mid;
+            else hi = mid - 1;
+        }
+        inst = lo;
+    }
+    P.inst = inst;
+    const int64_t local = path - (J.n_inst > 1 ? J.inst_first[inst] : 0);
+    P.seed = J.seed[inst];
+    P.pid = J.pid0[inst] + (uint64_t)local;
+    P.gamma = J.gamma[inst];
+    const int j = role < 7 ? role + 1 : 7;
+    const cx<double> *f = J.fpm + (int64_t)inst * 30;
+    P.f6[0] = f[0];
+    P.f6[1] = f[j];
+    P.f6[2] = f[8];
+    P.f6[3] = f[8 + j];
+    P.f6[4] = f[15 + j];
+    P.f6[5] = f[22 + j];
+    if (J.spt) {
+        const cx<double> *zs = J.spt + path * 21;
+        const cx<double> *qs = J.spm + path * 30;
+        P.s6[0] = qs[0];
+        P.s6[1] = qs[j];
+        P.s6[2] = qs[8];
+        P.s6[3] = qs[8 + j];
+        P.s6[4] = qs[15 + j];
+        P.s6[5] = qs[22 + j];
+        if (role < 7) {
+            P.Yc[0] = zs[2 * role];
+            P.Yc[1] = zs[2 * role + 1];
+            P.Yc[2] = zs[14 + role];
+        } else {
+            P.Yc[0] = mk(1.0, 0.0);
+            P.Yc[1] = mk(0.0, 0.0);
+            P.Yc[2] = mk(0.0, 0.0);
+        }
+    } else {
+        generate_start(role, o, P);
+    }
+#pragma unroll
+    for (int p = 0; p < 6; ++p) P.d6[p] = P.f6[p] - P.s6[p];
+    // TrackPath reads the same stream from draw 0 (frg.cu:54,57)
+    draw_vec(role, P, 0, o.patch_lo, o.patch_hi, P.u);
+    switch_patch(role, P.u, P.Yc);
+    draw_vec(role, P, 46, o.patch_lo, o.patch_hi, P.Vc);
+    P.draw = 92;
+    P.phase = PH_INIT;
+    P.stage = 0;
+    P.code = 0;
+    P.done = false;
+    P.s = o.sinit;
+    P.tstep = false;
+    P.streak = o.caution_steps - 1;
+    P.step = 0;
+    P.ministep = 0;
+    P.err = 0.0;
+    P.scale = 1.0;
+    P.residual = 100.0;
+    P.attempt = 0;
+    P.its = 0;
+    P.tgoal = 0.0;
+    P.st_rk = P.st_newton_calls = P.st_newton_its = P.st_redraws = 0;
+    P.st_ivp = 0;
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) P.K[m][k] = mk(0.0, 0.0);
+}
+
+// ------------------------------------------------------------------------------------------
+// State-machine transitions (htpytrack.cu:83-216).  Each returns with the next solve set up or
+// with P.done set.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void start_rk(PathState &P) {
+    P.phase = PH_RK;
+    P.stage = 0;
+}
+__device__ __forceinline__ void main_loop_top(const TrackOpts &o, PathState &P) {
+    if (P.step == o.max_steps) {  // :85
+        P.code = 1;
+        P.done = true;
+        return;
+    }
+    start_rk(P);
+}
+__device__ __forceinline__ void start_newton(const TrackOpts &o, PathState &P, double tgoal, bool sharp) {
+    // YN = Yn (K[1] -> K[0])
+#pragma unroll
+    for (int k = 0; k < 3; ++k) P.K[0][k] = P.K[1][k];
+    P.its = 0;
+    P.residual = 100.0;
+    P.tgoal = tgoal;
+    P.phase = sharp ? PH_SHARP : PH_NEWTON;
+}
+__device__ __forceinline__ void redraw_patch(int role, const TrackOpts &o, PathState &P) {
+    draw_vec(role, P, P.draw, o.patch_lo, o.patch_hi, P.u);
+    P.draw += 46;
+    P.st_redraws++;
+    switch_patch(role, P.u, P.K[1]);  // only Yn is moved to the new patch (:151, :210)
+}
+__device__ __forceinline__ void start_sharpen(const TrackOpts &o, PathState &P) {
+    P.attempt = 0;
+    start_newton(o, P, 1.0, true);
+}
+// corrector entry (:133-135)
+__device__ __forceinline__ void correct(int role, const TrackOpts &o, PathState &P, double tn) {
+    if (fabs(1. - tn) < o.sharp_trigger) {
+        start_sharpen(o, P);
+        return;
+    }
+    P.attempt = 0;
+    start_newton(o, P, tn, false);
+}
+// end of an RK attempt (:94-123)
+__device__ __forceinline__ void finish_rk(int role, const TrackOpts &o, PathState &P) {
+    P.st_rk++;
+    const double s = P.s;
+    cx<double> lte[3], yn[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        cx<double> e = mk(0.0, 0.0), b = mk(0.0, 0.0);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            if (m == 1) continue;
+            e = e + (kE[m] * s) * P.K[m][k];
+            b = b + (kB[m] * s) * P.K[m][k];
+        }
+        lte[k] = e;
+        yn[k] = P.Yc[k] + b;
+    }
+    const double err = grp_norm(lte);
+    double scale = err > 10e-17 ? 0.8 * pow(o.rk_tol / err, 1. / 6.) : 5.0;
+    if (scale > 5) scale = 5;
+    P.err = err;
+    P.scale = scale;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) P.K[1][k] = yn[k];  // Yn (k2 is not used after the attempt)
+    if (P.ministep++ == o.max_ministeps) {  // :95
+        P.code = 2;
+        P.done = true;
+        return;
+    }
+    if (!(err <= o.rk_tol)) {  // rejected (:97-105)
+        if (P.s <= o.smin) {
+            P.code = 3;
+            P.done = true;
+            return;
+        }
+        P.s *= scale;
+        if (P.s < o.smin) P.s = o.smin * 0.9;
+        P.streak = 0;
+        start_rk(P);
+        return;
+    }
+    const double tn = grp_t(role, P.K[1]);
+    if (tn <= 1. - o.nudge_trigger || fabs(1. - tn) < o.sharp_trigger) {  // :115-118
+        correct(role, o, P, tn);
+        return;
+    }
+    P.tstep = true;  // land on t = 1 (:121-123)
+    P.s = 1 - grp_t(role, P.Yc);
+    start_rk(P);
+}
+// end of a corrector Newton call (:140-164)
+__device__ __forceinline__ void finish_newton(int role, const TrackOpts &o, PathState &P) {
+    P.st_newton_calls++;
+    if (P.ministep++ == o.max_ministeps) {
+        P.code = 2;
+        P.done = true;
+        return;
+    }
+    if (P.residual < o.newt_tol) {  // accept (:176-183)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            P.Yc[k] = P.K[0][k];
+            P.Vc[k] = P.K[7][k];
+        }
+        P.streak++;
+        if (P.streak < o.caution_steps && P.scale > 1.) P.scale = 1.;
+        P.s *= P.scale;
+        if (P.s > o.smax) P.s = o.smax;
+        else if (P.s < o.smin) P.s = o.smin * 0.9;
+        P.step++;
+        main_loop_top(o, P);
+        return;
+    }
+    redraw_patch(role, o, P);
+    P.attempt++;
+    if (P.attempt < o.newt_attempts) {
+        start_newton(o, P, grp_t(role, P.K[1]), false);
+        return;
+    }
+    if (P.s <= o.smin) {  // :159
+        P.code = 4;
+        P.done = true;
+        return;
+    }
+    P.s *= 0.5;
+    if (P.s < o.smin) P.s = o.smin * 0.9;
+    P.streak = 0;
+    start_rk(P);
+}
+// end of a sharpening Newton call (:198-216)
+__device__ __forceinline__ void finish_sharp(int role, const TrackOpts &o, PathState &P) {
+    P.st_newton_calls++;
+    if (P.ministep++ == o.max_ministeps) {
+        P.code = 2;
+        P.done = true;
+        return;
+    }
+    if (P.residual < o.sharp_tol) {
+        P.code = 0;
+        P.done = true;
+        return;
+    }
+    redraw_patch(role, o, P);
+    P.attempt++;
+    if (P.attempt < o.sharp_attempts) {
+        start_newton(o, P, 1.0, true);
+        return;
+    }
+    P.code = 5;
+    P.done = true;
+}
+
+// ------------------------------------------------------------------------------------------
+// Output: dehomogenise (frg.cu:60), CognateSolns (frgkinematics.cu:227-285), store (frg.cu:68).
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void cognate12(const cx<double> o[7], cx<double> P, cx<double> c[7]) {
+    const cx<double> A = o[0], B = o[1], C = o[2], D = o[3], F = o[4], G = o[5], H = o[6];
+    cx<double> sc = cdiv(B - F, D - F);
+    const cx<double> Dp = sc * (D - H) + H, Gp = sc * (G - H) + H;
+    sc = cdiv(P - F, H - F);
+    c[0] = sc * (A - B) + B;
+    c[1] = B;
+    c[2] = sc * (C - B) + B;
+    c[3] = sc * (Dp - B) + B;
+    c[4] = B + (P - F);
+    c[5] = sc * (Gp - B) + B;
+    c[6] = sc * (H - B) + B;
+}
+
+template <typename R>
+__device__ __forceinline__ void finish_path(int role, const Job &J, PathState &P, GroupLds<R> *L) {
+    // output point: Yc for codes 1-4 and code 2 before sharpening, the sharpened point otherwise
+    const bool from_sharp = (P.phase == PH_SHARP);
+    cx<double> Y[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) Y[k] = from_sharp ? P.K[0][k] : P.Yc[k];
+    const int64_t path = P.path;
+    if (J.yfinal) {
+        cx<double> *yo = J.yfinal + path * 23;
+        if (role < 7) {
+            yo[2 * role] = Y[0];
+            yo[2 * role + 1] = Y[1];
+            yo[14 + role] = Y[2];
+        } else {
+            yo[21] = Y[0];
+            yo[22] = Y[1];
+        }
+    }
+    // dehomogenise: divide z and Z0 by Z0
+    const cx<double> Z0 = grp_bcast(Y[0], role, 7);
+    const cx<double> iZ0 = inv(Z0);
+    cx<double> zd[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) zd[k] = Y[k] * iZ0;
+    // gather the 14 linkage values through LDS (as doubles, two LDS cells each)
+    cx<R> *cell = L->a;
+    double *dcell = reinterpret_cast<double *>(&L->phat[0][0]);
+    lds_sync();
+    if (role < 7) {
+        dcell[4 * role + 0] = zd[0].re;
+        dcell[4 * role + 1] = zd[0].im;
+        dcell[4 * role + 2] = zd[1].re;
+        dcell[4 * role + 3] = zd[1].im;
+    }
+    lds_sync();
+    cx<double> lk[14];
+#pragma unroll
+    for (int i = 0; i < 14; ++i) lk[i] = mk(dcell[2 * i], dcell[2 * i + 1]);
+    lds_sync();
+    (void)cell;
+    const cx<double> P0 = J.fpm[(int64_t)P.inst * 30 + 0], Pc0 = J.fpm[(int64_t)P.inst * 30 + 8];
+    const cx<double> Pj = P.f6[1];
+    cx<double> c1[7], c2[7];
+    cognate12(&lk[0], P0, c1);
+    cognate12(&lk[7], Pc0, c2);
+    const cx<double> B = lk[1], F = lk[4];
+    const cx<double> Udown = cdiv((Pj - B) - zd[2] * (P0 - F), F - B);
+    const bool swap = c1[0].re < lk[0].re;
+    if (role < 7) {
+        cx<double> *out = J.fpt + path * 21;
+        cx<double> cog[14];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+            cog[i] = c1[i];
+            cog[7 + i] = c2[i];
+        }
+        // this lane's two linkage entries
+        cx<double> o0 = zd[0], o1 = zd[1];
+#pragma unroll
+        for (int i = 0; i < 14; ++i) {
+            o0 = (swap && i == 2 * role) ? cog[i] : o0;
+            o1 = (swap && i == 2 * role + 1) ? cog[i] : o1;
+        }
+        out[2 * role] = o0;
+        out[2 * role + 1] = o1;
+        out[14 + role] = swap ? Udown : zd[2];
+    }
+    if (role == 0) {
+        J.err[path] = P.code;
+        if (J.stats) {
+            int *st = J.stats + path * 8;
+            st[0] = P.step;
+            st[1] = P.st_rk;
+            st[2] = P.st_newton_calls;
+            st[3] = P.st_newton_its;
+            st[4] = P.ministep;
+            st[5] = P.st_redraws;
+            st[6] = P.st_ivp;
+            st[7] = P.code;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// One solve for the current phase, then the transition.
+// ------------------------------------------------------------------------------------------
+template <typename R>
+__device__ __forceinline__ void micro_step(int role, const TrackOpts &o, PathState &P, GroupLds<R> *L) {
+    // evaluation point, w row and right-hand side of the current phase
+    const bool newton = (P.phase == PH_NEWTON || P.phase == PH_SHARP);
+    const bool rk = (P.phase == PH_RK);
+    cx<double> Ye[3], W[3];
+    double w_rhs = 1.0;
+    {
+        double c[7];
+        const double s = P.s;
+#pragma unroll
+        for (int m = 0; m < 7; ++m) c[m] = rk ? rk_a(P.stage, m) * s : 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            cx<double> acc = P.Yc[k];
+#pragma unroll
+            for (int m = 0; m < 7; ++m) acc = acc + c[m] * P.K[m][k];
+            Ye[k] = newton ? P.K[0][k] : acc;
+            // w: Vc at init and stage 0, k_{stage-1} otherwise; e_t for Newton
+            cx<double> wv = P.Vc[k];
+#pragma unroll
+            for (int m = 0; m < 7; ++m) wv = (rk && P.stage == m + 1) ? P.K[m][k] : wv;
+            const bool is_t = (role == 7 && k == 1);
+            W[k] = newton ? (is_t ? mk(1.0, 0.0) : mk(0.0, 0.0)) : wv;
+        }
+    }
+    double tcur = 0.0;
+    if (newton) {
+        tcur = grp_t(role, Ye);
+        w_rhs = P.tgoal - tcur;
+    }
+    // FP64 residual of this lane's equations (Newton only)
+    cx<double> fN[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+    if (newton && sizeof(R) != sizeof(double)) {
+        // gather the point in FP64 through LDS (two cells per value)
+        double *dcell = reinterpret_cast<double *>(&L->phat[0][0]);
+        lds_sync();
+        dcell[4 * role + 0] = Ye[0].re;
+        dcell[4 * role + 1] = Ye[0].im;
+        dcell[4 * role + 2] = Ye[1].re;
+        dcell[4 * role + 3] = Ye[1].im;
+        lds_sync();
+        PoseEval<double> e;
+#pragma unroll
+        for (int i = 0; i < 14; ++i) e.L[i] = mk(dcell[2 * i], dcell[2 * i + 1]);
+        e.Z0 = mk(dcell[28], dcell[29]);
+        e.t = dcell[30];
+        lds_sync();
+        e.U = role < 7 ? Ye[2] : mk(1.0, 0.0);
+#pragma unroll
+        for (int p = 0; p < 6; ++p) {
+            e.s[p] = P.s6[p];
+            e.d[p] = P.d6[p];
+        }
+        e.gamma = P.gamma;
+        pose_values(e, fN[0], fN[1], fN[2]);
+    }
+    cx<double> x[3];
+    group_solve<R>(role, L, Ye, P.u, W, newton, w_rhs, P.s6, P.d6, P.gamma, fN, x);
+    if (!newton) {
+        P.st_ivp++;
+        // tangent normalisation (htpysupport.cu:285-296)
+        const double nrm = grp_norm(x);
+        const cx<double> xt = grp_bcast(x[1], role, 7);
+        cx<double> sc;
+        if (!P.tstep || P.phase == PH_INIT) {
+            const double a = sqrt(abs2(xt));
+            sc = a > 0.0 ? mk(xt.re / (a * nrm), -xt.im / (a * nrm)) : mk(1.0 / nrm, 0.0);
+        } else {
+            sc = inv(xt);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) x[k] = sc * x[k];
+        if (role == 7) x[1].im = 0.0;
+        if (P.phase == PH_INIT) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) P.Vc[k] = x[k];
+            main_loop_top(o, P);
+            return;
+        }
+        // store k_{stage+1}
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) P.K[m][k] = (P.stage == m) ? x[k] : P.K[m][k];
+        P.stage++;
+        if (P.stage == 8) finish_rk(role, o, P);
+        return;
+    }
+    // Newton update (htpysupport.cu:475-491)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) P.K[0][k] = P.K[0][k] + x[k];
+    P.residual = grp_norm(x);
+    P.its++;
+    P.st_newton_its++;
+    const int max_its = P.phase == PH_SHARP ? o.sharp_its : o.newt_its;
+    const double tol = P.phase == PH_SHARP ? o.sharp_tol : o.newt_tol;
+    if (P.its < max_its && P.residual > tol) return;  // another iteration
+    if (P.phase == PH_SHARP) finish_sharp(role, o, P);
+    else finish_newton(role, o, P);
+}
+
+// Next path for a persistent group: lane 0 takes a ticket from the device queue.
+__device__ __forceinline__ int64_t claim_path(int role, unsigned long long *queue) {
+    unsigned long long v = 0;
+    if (role == 0) v = atomicAdd(queue, 1ull);
+    const int lo = grp_bcast_i((int)(uint32_t)v, role, 0);
+    const int hi = grp_bcast_i((int)(uint32_t)(v >> 32), role, 0);
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Wave-uniform precision choice for mixed mode: FP32 solve unless some group of the wave needs
+// FP64 (initial tangent, sharpening, or within fp64_band of t = 1).
+__device__ __forceinline__ bool needs_fp64(int role, const TrackOpts &o, const PathState &P) {
+    if (o.precision == 0) return true;
+    if (P.phase == PH_INIT || P.phase == PH_SHARP) return true;
+    const double tc = grp_t(role, P.Yc);
+    return (1.0 - tc) < o.fp64_band;
+}
+
+// ------------------------------------------------------------------------------------------
+// Kernels.  Static: group g of the grid tracks path g (the reference's one-thread-per-path
+// launch, frg.cu:374).  Persistent: a fixed grid; groups take paths from the queue until it
+// runs dry.
+// ------------------------------------------------------------------------------------------
+template <bool kPersistent, bool kMixed>
+__global__ __launch_bounds__(256) void track_kernel(Job J) {
+    extern __shared__ unsigned char smem[];
+    const int role = threadIdx.x & 7;
+    const int grp = threadIdx.x >> 3;
+    GroupLds<double> *L = reinterpret_cast<GroupLds<double> *>(smem) + grp;
+    PathState P;
+    int64_t path = kPersistent ? claim_path(role, J.queue) : (int64_t)blockIdx.x * (blockDim.x >> 3) + grp;
+    if (path >= J.n_total) return;
+    init_path(role, J, P, path);
+    for (;;) {
+        if (kMixed) {
+            const bool need64 = needs_fp64(role, J.o, P);
+            if (__any(need64)) micro_step<double>(role, J.o, P, L);
+            else micro_step<float>(role, J.o, P, reinterpret_cast<GroupLds<float> *>(L));
+        } else {
+            micro_step<double>(role, J.o, P, L);
+        }
+        if (P.done) {
+            finish_path(role, J, P, L);
+            if (!kPersistent) break;
+            path = claim_path(role, J.queue);
+            if (path >= J.n_total) break;
+            init_path(role, J, P, path);
+        }
+    }
+}
+
+size_t track_lds_bytes(int block, int precision) {
+    (void)precision;
+    return (size_t)(block / 8) * sizeof(GroupLds<double>);
+}
+
+int track_max_groups_per_cu(int block, int kind, int precision) {
+    int nb = 0;
+    const size_t lds = track_lds_bytes(block, precision);
+    hipError_t e;
+    if (kind == LK_PERSISTENT) {
+        e = precision ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, track_kernel<true, true>, block, lds)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, track_kernel<true, false>, block, lds);
+    } else {
+        e = precision ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, track_kernel<false, true>, block, lds)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, track_kernel<false, false>, block, lds);
+    }
+    if (e != hipSuccess) return 0;
+    return nb * (block / 8);
+}
+
+hipError_t launch_track(const Job &job, int kind, int block, int grid, hipStream_t stream) {
+    if (block % 64 != 0 || block < 64 || block > 256) return hipErrorInvalidValue;
+    const size_t lds = track_lds_bytes(block, job.o.precision);
+    const bool mixed = job.o.precision != 0;
+    if (kind == LK_PERSISTENT) {
+        if (mixed) hipLaunchKernelGGL((track_kernel<true, true>), dim3(grid), dim3(block), lds, stream, job);
+        else hipLaunchKernelGGL((track_kernel<true, false>), dim3(grid), dim3(block), lds, stream, job);
+    } else {
+        if (mixed) hipLaunchKernelGGL((track_kernel<false, true>), dim3(grid), dim3(block), lds, stream, job);
+        else hipLaunchKernelGGL((track_kernel<false, false>), dim3(grid), dim3(block), lds, stream, job);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace frg
