@@ -1,0 +1,459 @@
+// frg_capi.cpp -- the C ABI of include/frg.h over the HIP runtime.
+//
+// Owns: the device, one stream (default priority), device buffers grown on demand, and the
+// launch-shape choice per (schedule, precision).  Replaces the host side of the reference's
+// round (frg.cu:138-236 set-up, :356-404 launch and copies, :518-531 tear-down).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/frg.h"
+#include "frg_kernels.h"
+
+using frg::cx;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, need);
+        if (e == hipSuccess) bytes = need;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+}  // namespace
+
+struct frg_ctx {
+    int device = 0;
+    int num_cu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    frg_opts opts{};
+    DevBuf inst_first, fpm, gamma, seed, pid0, spt, spm, fpt, err, stats, yfinal, queue, dbg_a, dbg_b, dbg_c,
+        dbg_d, dbg_e, dbg_x;
+};
+
+namespace {
+
+#define HIPCHK(x)                              \
+    do {                                       \
+        hipError_t e_ = (x);                   \
+        if (e_ != hipSuccess) return FRG_EHIP; \
+    } while (0)
+
+frg::TrackOpts to_track_opts(const frg_opts &o) {
+    frg::TrackOpts t{};
+    t.sinit = o.sinit;
+    t.smax = o.smax;
+    t.smin = o.smin;
+    t.rk_tol = o.rk_tol;
+    t.caution_steps = o.caution_steps;
+    t.newt_its = o.newt_its;
+    t.newt_tol = o.newt_tol;
+    t.newt_attempts = o.newt_attempts;
+    t.sharp_its = o.sharp_its;
+    t.sharp_tol = o.sharp_tol;
+    t.sharp_attempts = o.sharp_attempts;
+    t.sharp_trigger = o.sharp_trigger;
+    t.nudge_trigger = o.nudge_trigger;
+    t.max_steps = o.max_steps;
+    t.max_ministeps = o.max_ministeps;
+    t.patch_lo = o.patch_lo;
+    t.patch_hi = o.patch_hi;
+    t.start_lo = o.start_lo;
+    t.start_hi = o.start_hi;
+    t.start_mode = o.start_mode;
+    t.precision = o.precision;
+    t.fp64_band = o.fp64_band;
+    return t;
+}
+
+int check_opts(const frg_opts &o) {
+    if (o.newt_residual != 1) return FRG_EUNSUPPORTED;
+    if (o.start_mode != FRG_START_UNIFORM && o.start_mode != FRG_START_GAUSSIAN) return FRG_EINVAL;
+    if (o.precision != FRG_PREC_FP64 && o.precision != FRG_PREC_MIXED) return FRG_EINVAL;
+    if (o.schedule != FRG_SCHED_STATIC && o.schedule != FRG_SCHED_PERSISTENT) return FRG_EINVAL;
+    if (o.step_control != FRG_STEP_CLASSIC) return FRG_EUNSUPPORTED;
+    if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 256 || o.block_size % 64 != 0)) return FRG_EINVAL;
+    if (o.groups_per_cu < 0 || o.max_steps < 0 || o.max_ministeps < 0) return FRG_EINVAL;
+    if (!(o.smin > 0.0) || !(o.rk_tol > 0.0)) return FRG_EINVAL;
+    return FRG_OK;
+}
+
+// Default launch shapes: 64-thread blocks (8 paths per block); persistent grids fill every CU
+// to the occupancy the kernel's register use allows.
+int pick_block(const frg_ctx *c) { return c->opts.block_size ? c->opts.block_size : 64; }
+
+int run_job(frg_ctx *c, frg::Job &job, hipStream_t stream, frg_stats *summary) {
+    job.o = to_track_opts(c->opts);
+    const int block = pick_block(c);
+    const int gpb = block / 8;
+    int grid;
+    int kind = c->opts.schedule == FRG_SCHED_PERSISTENT ? frg::LK_PERSISTENT : frg::LK_STATIC;
+    if (kind == frg::LK_PERSISTENT) {
+        int per_cu = c->opts.groups_per_cu;
+        if (per_cu == 0) per_cu = frg::track_max_groups_per_cu(block, kind, c->opts.precision);
+        if (per_cu < gpb) per_cu = gpb;
+        grid = std::max(1, (per_cu / gpb) * c->num_cu);
+        const int64_t need = (job.n_total + gpb - 1) / gpb;
+        if ((int64_t)grid > need) grid = (int)need;
+        HIPCHK(c->queue.ensure(sizeof(unsigned long long)));
+        job.queue = c->queue.as<unsigned long long>();
+        HIPCHK(frg::launch_reset_counter(job.queue, stream));
+    } else {
+        const int64_t g = (job.n_total + gpb - 1) / gpb;
+        if (g > 0x7fffffff) return FRG_EINVAL;
+        grid = (int)g;
+    }
+    if (job.n_total == 0) return FRG_OK;
+    HIPCHK(hipEventRecord(c->ev0, stream));
+    HIPCHK(frg::launch_track(job, kind, block, grid, stream));
+    HIPCHK(hipEventRecord(c->ev1, stream));
+    if (summary) {
+        summary->block_size = block;
+        summary->grid_size = grid;
+        summary->schedule = c->opts.schedule;
+        summary->precision = c->opts.precision;
+    }
+    return FRG_OK;
+}
+
+void fill_summary_counts(frg_stats *s, int64_t n, const int32_t *err, const int32_t *st) {
+    s->n_paths = n;
+    for (int k = 0; k < 8; ++k) s->code_count[k] = 0;
+    s->steps = s->rk_attempts = s->newton_calls = s->newton_its = s->tangent_solves = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const int e = err[i];
+        if (e >= 0 && e < 8) s->code_count[e]++;
+        if (st) {
+            const int32_t *r = st + i * 8;
+            s->steps += r[0];
+            s->rk_attempts += r[1];
+            s->newton_calls += r[2];
+            s->newton_its += r[3];
+            s->tangent_solves += r[6];
+        }
+    }
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Uploads the single-instance description.
+int upload_instances(frg_ctx *c, int32_t n_inst, const frg_c64 *fpm, const frg_c64 *gamma, const uint64_t *seeds,
+                     const uint64_t *pid0s, const int64_t *counts, frg::Job &job, hipStream_t stream) {
+    std::vector<int64_t> first(n_inst + 1, 0);
+    for (int i = 0; i < n_inst; ++i) {
+        if (counts[i] < 0) return FRG_EINVAL;
+        first[i + 1] = first[i] + counts[i];
+    }
+    HIPCHK(c->inst_first.ensure(sizeof(int64_t) * (n_inst + 1)));
+    HIPCHK(c->fpm.ensure(sizeof(frg_c64) * 30 * n_inst));
+    HIPCHK(c->gamma.ensure(sizeof(frg_c64) * n_inst));
+    HIPCHK(c->seed.ensure(sizeof(uint64_t) * n_inst));
+    HIPCHK(c->pid0.ensure(sizeof(uint64_t) * n_inst));
+    HIPCHK(hipMemcpyAsync(c->inst_first.p, first.data(), sizeof(int64_t) * (n_inst + 1), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(c->fpm.p, fpm, sizeof(frg_c64) * 30 * n_inst, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(c->gamma.p, gamma, sizeof(frg_c64) * n_inst, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(c->seed.p, seeds, sizeof(uint64_t) * n_inst, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(c->pid0.p, pid0s, sizeof(uint64_t) * n_inst, hipMemcpyHostToDevice, stream));
+    // the host vector dies at return: make sure the copies have consumed it
+    HIPCHK(hipStreamSynchronize(stream));
+    job.n_total = first[n_inst];
+    job.n_inst = n_inst;
+    job.inst_first = c->inst_first.as<int64_t>();
+    job.fpm = c->fpm.as<cx<double>>();
+    job.gamma = c->gamma.as<cx<double>>();
+    job.seed = c->seed.as<uint64_t>();
+    job.pid0 = c->pid0.as<uint64_t>();
+    return FRG_OK;
+}
+
+// Runs a job whose outputs live in the context's buffers and copies them to the host.
+int run_host(frg_ctx *c, frg::Job &job, frg_c64 *fpt, int32_t *err, int32_t *path_stats, frg_c64 *yfinal,
+             frg_stats *summary, std::chrono::steady_clock::time_point t0) {
+    const int64_t n = job.n_total;
+    const bool want_stats = path_stats || summary;
+    HIPCHK(c->fpt.ensure(sizeof(frg_c64) * 21 * std::max<int64_t>(n, 1)));
+    HIPCHK(c->err.ensure(sizeof(int32_t) * std::max<int64_t>(n, 1)));
+    job.fpt = c->fpt.as<cx<double>>();
+    job.err = c->err.as<int>();
+    job.stats = nullptr;
+    job.yfinal = nullptr;
+    if (want_stats) {
+        HIPCHK(c->stats.ensure(sizeof(int32_t) * 8 * std::max<int64_t>(n, 1)));
+        job.stats = c->stats.as<int>();
+    }
+    if (yfinal) {
+        HIPCHK(c->yfinal.ensure(sizeof(frg_c64) * 23 * std::max<int64_t>(n, 1)));
+        job.yfinal = c->yfinal.as<cx<double>>();
+    }
+    int rc = run_job(c, job, c->stream, summary);
+    if (rc != FRG_OK) return rc;
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(fpt, c->fpt.p, sizeof(frg_c64) * 21 * n, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(err, c->err.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+        if (yfinal)
+            HIPCHK(hipMemcpyAsync(yfinal, c->yfinal.p, sizeof(frg_c64) * 23 * n, hipMemcpyDeviceToHost, c->stream));
+    }
+    std::vector<int32_t> tmp;
+    int32_t *st = path_stats;
+    if (want_stats && n > 0) {
+        if (!st) {
+            tmp.resize((size_t)n * 8);
+            st = tmp.data();
+        }
+        HIPCHK(hipMemcpyAsync(st, c->stats.p, sizeof(int32_t) * 8 * n, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (summary) {
+        fill_summary_counts(summary, n, err, st);
+        float ms = 0.f;
+        if (n > 0) HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        summary->kernel_ms = ms;
+        summary->total_ms = ms_since(t0);
+    }
+    return FRG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int frg_version(void) { return 10000; }
+
+const char *frg_strerror(int code) {
+    switch (code) {
+        case FRG_OK: return "ok";
+        case FRG_EINVAL: return "invalid argument";
+        case FRG_EHIP: return "HIP runtime error";
+        case FRG_ENOMEM: return "out of memory";
+        case FRG_EUNSUPPORTED: return "unsupported option";
+        default: return "unknown error";
+    }
+}
+
+int frg_default_opts(frg_opts *o) {
+    if (!o) return FRG_EINVAL;
+    std::memset(o, 0, sizeof(*o));
+    o->sinit = 1.;
+    o->smax = 100.;
+    o->smin = 0.000001;
+    o->rk_tol = 0.001;
+    o->caution_steps = 2;
+    o->newt_its = 10;
+    o->newt_tol = 0.00000001;
+    o->newt_attempts = 3;
+    o->sharp_its = 30;
+    o->sharp_tol = 0.000000000001;
+    o->sharp_attempts = 3;
+    o->newt_residual = 1;
+    o->sharp_trigger = 0.00001;
+    o->nudge_trigger = 0.00001;
+    o->max_steps = 2000;
+    o->max_ministeps = 3000;
+    o->patch_lo = -2.;
+    o->patch_hi = 2.;
+    o->start_lo = -5.;
+    o->start_hi = 5.;
+    o->start_mode = FRG_START_UNIFORM;
+    o->precision = FRG_PREC_FP64;
+    o->fp64_band = 1e-2;
+    o->schedule = FRG_SCHED_STATIC;
+    o->block_size = 0;
+    o->groups_per_cu = 0;
+    o->step_control = FRG_STEP_CLASSIC;
+    return FRG_OK;
+}
+
+int frg_create(frg_ctx **out, int device, const frg_opts *opts) {
+    if (!out) return FRG_EINVAL;
+    *out = nullptr;
+    frg_opts o;
+    if (opts) o = *opts;
+    else frg_default_opts(&o);
+    int rc = check_opts(o);
+    if (rc != FRG_OK) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return FRG_EINVAL;
+    frg_ctx *c = new (std::nothrow) frg_ctx();
+    if (!c) return FRG_ENOMEM;
+    c->device = device;
+    c->opts = o;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        frg_destroy(c);
+        return FRG_EHIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        frg_destroy(c);
+        return FRG_EHIP;
+    }
+    c->num_cu = prop.multiProcessorCount;
+    *out = c;
+    return FRG_OK;
+}
+
+void frg_destroy(frg_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    DevBuf *bufs[] = {&c->inst_first, &c->fpm, &c->gamma, &c->seed, &c->pid0, &c->spt, &c->spm, &c->fpt, &c->err,
+                      &c->stats, &c->yfinal, &c->queue, &c->dbg_a, &c->dbg_b, &c->dbg_c, &c->dbg_d, &c->dbg_e,
+                      &c->dbg_x};
+    for (DevBuf *b : bufs) b->release();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int frg_set_opts(frg_ctx *c, const frg_opts *o) {
+    if (!c || !o) return FRG_EINVAL;
+    int rc = check_opts(*o);
+    if (rc != FRG_OK) return rc;
+    c->opts = *o;
+    return FRG_OK;
+}
+
+int frg_track(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id, int64_t n_paths,
+              frg_c64 *fpt, int32_t *err, int32_t *path_stats, frg_stats *summary) {
+    if (!c || !fpm || n_paths < 0 || (n_paths > 0 && (!fpt || !err))) return FRG_EINVAL;
+    auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipSetDevice(c->device));
+    frg::Job job{};
+    int rc = upload_instances(c, 1, fpm, &gamma, &seed, &first_path_id, &n_paths, job, c->stream);
+    if (rc != FRG_OK) return rc;
+    return run_host(c, job, fpt, err, path_stats, nullptr, summary, t0);
+}
+
+int frg_track_device(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
+                     int64_t n_paths, frg_c64 *d_fpt, int32_t *d_err, int32_t *d_path_stats, void *stream) {
+    if (!c || !fpm || n_paths < 0 || (n_paths > 0 && (!d_fpt || !d_err))) return FRG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    frg::Job job{};
+    int rc = upload_instances(c, 1, fpm, &gamma, &seed, &first_path_id, &n_paths, job, s);
+    if (rc != FRG_OK) return rc;
+    job.fpt = reinterpret_cast<cx<double> *>(d_fpt);
+    job.err = d_err;
+    job.stats = d_path_stats;
+    job.yfinal = nullptr;
+    return run_job(c, job, s, nullptr);
+}
+
+int frg_track_explicit(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, const frg_c64 *spt, const frg_c64 *spm,
+                       uint64_t seed, uint64_t first_path_id, int64_t n_paths, frg_c64 *fpt, int32_t *err,
+                       int32_t *path_stats, frg_c64 *yfinal, frg_stats *summary) {
+    if (!c || !fpm || n_paths < 0 || (n_paths > 0 && (!spt || !spm || !fpt || !err))) return FRG_EINVAL;
+    auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipSetDevice(c->device));
+    frg::Job job{};
+    int rc = upload_instances(c, 1, fpm, &gamma, &seed, &first_path_id, &n_paths, job, c->stream);
+    if (rc != FRG_OK) return rc;
+    if (n_paths > 0) {
+        HIPCHK(c->spt.ensure(sizeof(frg_c64) * 21 * n_paths));
+        HIPCHK(c->spm.ensure(sizeof(frg_c64) * 30 * n_paths));
+        HIPCHK(hipMemcpyAsync(c->spt.p, spt, sizeof(frg_c64) * 21 * n_paths, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->spm.p, spm, sizeof(frg_c64) * 30 * n_paths, hipMemcpyHostToDevice, c->stream));
+        job.spt = c->spt.as<cx<double>>();
+        job.spm = c->spm.as<cx<double>>();
+    }
+    return run_host(c, job, fpt, err, path_stats, yfinal, summary, t0);
+}
+
+int frg_track_batch(frg_ctx *c, int32_t n_inst, const frg_c64 *fpm, const frg_c64 *gamma, const uint64_t *seeds,
+                    const uint64_t *first_path_ids, const int64_t *paths_per_inst, frg_c64 *fpt, int32_t *err,
+                    int32_t *path_stats, frg_stats *summary) {
+    if (!c || n_inst <= 0 || !fpm || !gamma || !seeds || !first_path_ids || !paths_per_inst) return FRG_EINVAL;
+    auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipSetDevice(c->device));
+    frg::Job job{};
+    int rc = upload_instances(c, n_inst, fpm, gamma, seeds, first_path_ids, paths_per_inst, job, c->stream);
+    if (rc != FRG_OK) return rc;
+    if (job.n_total > 0 && (!fpt || !err)) return FRG_EINVAL;
+    return run_host(c, job, fpt, err, path_stats, nullptr, summary, t0);
+}
+
+int frg_generate_starts(frg_ctx *c, uint64_t seed, uint64_t first_path_id, int64_t n, frg_c64 *spt, frg_c64 *spm) {
+    if (!c || n < 0 || (n > 0 && (!spt || !spm))) return FRG_EINVAL;
+    if (n == 0) return FRG_OK;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(c->spt.ensure(sizeof(frg_c64) * 21 * n));
+    HIPCHK(c->spm.ensure(sizeof(frg_c64) * 30 * n));
+    frg::DebugStartArgs a{};
+    a.n = n;
+    a.seed = seed;
+    a.pid0 = first_path_id;
+    a.o = to_track_opts(c->opts);
+    a.spt = c->spt.as<cx<double>>();
+    a.spm = c->spm.as<cx<double>>();
+    HIPCHK(frg::launch_debug_start(a, c->stream));
+    HIPCHK(hipMemcpyAsync(spt, c->spt.p, sizeof(frg_c64) * 21 * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(spm, c->spm.p, sizeof(frg_c64) * 30 * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FRG_OK;
+}
+
+int frg_solve_step(frg_ctx *c, int32_t mode, int32_t precision, int64_t n, const frg_c64 *Y, const frg_c64 *u,
+                   const frg_c64 *w, const frg_c64 *spm, const frg_c64 *fpm, frg_c64 gamma, const double *tgoal,
+                   frg_c64 *x) {
+    if (!c || n < 0 || mode < 0 || mode > 2) return FRG_EINVAL;
+    if (precision != FRG_PREC_FP64 && precision != FRG_PREC_MIXED) return FRG_EINVAL;
+    if (n == 0) return FRG_OK;
+    if (!Y || !u || !spm || !fpm || !x || (mode < 2 && !w) || (mode == 2 && !tgoal)) return FRG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    const size_t v23 = sizeof(frg_c64) * 23 * n;
+    HIPCHK(c->dbg_a.ensure(v23));
+    HIPCHK(c->dbg_b.ensure(v23));
+    HIPCHK(c->dbg_c.ensure(v23));
+    HIPCHK(c->dbg_d.ensure(sizeof(frg_c64) * 30 * n + sizeof(frg_c64) * 30));
+    HIPCHK(c->dbg_e.ensure(sizeof(double) * n));
+    HIPCHK(c->dbg_x.ensure(v23));
+    HIPCHK(hipMemcpyAsync(c->dbg_a.p, Y, v23, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dbg_b.p, u, v23, hipMemcpyHostToDevice, c->stream));
+    if (w) HIPCHK(hipMemcpyAsync(c->dbg_c.p, w, v23, hipMemcpyHostToDevice, c->stream));
+    else HIPCHK(hipMemsetAsync(c->dbg_c.p, 0, v23, c->stream));
+    HIPCHK(hipMemcpyAsync(c->dbg_d.p, spm, sizeof(frg_c64) * 30 * n, hipMemcpyHostToDevice, c->stream));
+    frg_c64 *dfpm = c->dbg_d.as<frg_c64>() + 30 * n;
+    HIPCHK(hipMemcpyAsync(dfpm, fpm, sizeof(frg_c64) * 30, hipMemcpyHostToDevice, c->stream));
+    if (tgoal) HIPCHK(hipMemcpyAsync(c->dbg_e.p, tgoal, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    else HIPCHK(hipMemsetAsync(c->dbg_e.p, 0, sizeof(double) * n, c->stream));
+    frg::DebugSolveArgs a{};
+    a.n = n;
+    a.mode = mode;
+    a.Y = c->dbg_a.as<cx<double>>();
+    a.u = c->dbg_b.as<cx<double>>();
+    a.w = c->dbg_c.as<cx<double>>();
+    a.spm = c->dbg_d.as<cx<double>>();
+    a.fpm = reinterpret_cast<cx<double> *>(dfpm);
+    a.tgoal = c->dbg_e.as<double>();
+    a.gamma = frg::mk(gamma.re, gamma.im);
+    a.x = c->dbg_x.as<cx<double>>();
+    HIPCHK(frg::launch_debug_solve(a, precision, c->stream));
+    HIPCHK(hipMemcpyAsync(x, c->dbg_x.p, v23, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FRG_OK;
+}
+
+}  // extern "C"

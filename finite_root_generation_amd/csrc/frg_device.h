@@ -22,7 +22,7 @@ struct cx {
 };
 
 template <typename R>
-__device__ __forceinline__ cx<R> mk(R a, R b) {
+__host__ __device__ __forceinline__ cx<R> mk(R a, R b) {
     cx<R> z;
     z.re = a;
     z.im = b;

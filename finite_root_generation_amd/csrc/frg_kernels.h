@@ -43,9 +43,29 @@ struct Job {
     TrackOpts o;
 };
 
+// Stage-level parity entry points (one group solve per path; start generation alone).
+struct DebugSolveArgs {
+    int64_t n;
+    int mode;  // 0 arclength tangent, 1 t-step tangent, 2 Newton update
+    const cx<double> *Y, *u, *w, *spm, *fpm;  // n x 23, n x 23, n x 23, n x 30, 30
+    const double *tgoal;                       // n (Newton)
+    cx<double> gamma;
+    cx<double> *x;                             // n x 23 out
+};
+struct DebugStartArgs {
+    int64_t n;
+    uint64_t seed, pid0;
+    TrackOpts o;
+    cx<double> *spt, *spm;  // n x 21, n x 30 out
+};
+hipError_t launch_debug_solve(const DebugSolveArgs &a, int precision, hipStream_t stream);
+hipError_t launch_debug_start(const DebugStartArgs &a, hipStream_t stream);
+
 // host-side launchers (frg_tracker.hip)
 enum LaunchKind : int { LK_STATIC = 0, LK_PERSISTENT = 1 };
 hipError_t launch_track(const Job &job, int kind, int block, int grid, hipStream_t stream);
+// zeroes a work-queue counter with a kernel (capturable, unlike a memset node)
+hipError_t launch_reset_counter(unsigned long long *p, hipStream_t stream);
 size_t track_lds_bytes(int block, int precision);
 int track_max_groups_per_cu(int block, int kind, int precision);
 

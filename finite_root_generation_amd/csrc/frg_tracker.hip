@@ -120,7 +120,7 @@ __device__ __forceinline__ void group_solve(int role, GroupLds<R> *L, const cx<d
     um = grp_max(um);
     uy = grp_sum(uy);
     const double usc = 1.0 / um;
-    const cx<double> bu = newton ? mk(-(uy.re - 1.0) * usc, -uy.im * usc) : mk(0.0, 0.0);
+    const cx<double> bu = sel(newton, mk(-(uy.re - 1.0) * usc, -uy.im * usc), mk(0.0, 0.0));
 
     // -------- 1. broadcast the shared part of the evaluation point --------
     L->a[2 * role] = cvt<R>(Ye[0]);
@@ -132,7 +132,7 @@ __device__ __forceinline__ void group_solve(int role, GroupLds<R> *L, const cx<d
     e.Z0 = L->a[14];
     e.t = L->a[15].re;
     lds_sync();
-    e.U = role < 7 ? cvt<R>(Ye[2]) : mk(R(1), R(0));
+    e.U = sel(role < 7, cvt<R>(Ye[2]), mk(R(1), R(0)));
 #pragma unroll
     for (int p = 0; p < 6; ++p) {
         e.s[p] = cvt<R>(s6[p]);
@@ -230,8 +230,8 @@ __device__ __forceinline__ void group_solve(int role, GroupLds<R> *L, const cx<d
         L->a[16 + 2 * role + 1] = cw1;
     }
     // right-hand sides of the reduced patch rows: b - sum_j (U-entry_j * rho_j)
-    const cx<R> su = grp_sum(role < 7 ? uU * rho : zero);
-    const cx<R> sw = grp_sum(role < 7 ? wU * rho : zero);
+    const cx<R> su = grp_sum(sel(role < 7, uU * rho, zero));
+    const cx<R> sw = grp_sum(sel(role < 7, wU * rho, zero));
     lds_sync();
     if (role == 7) {
 #pragma unroll
@@ -270,15 +270,15 @@ __device__ __forceinline__ void group_solve(int role, GroupLds<R> *L, const cx<d
             invB = inv(rb[c]);
         }
         if (pA || pB) {
-            L->a[c] = pA ? invA : invB;
+            L->a[c] = sel(pA, invA, invB);
 #pragma unroll
             for (int k = 0; k < 17; ++k)
-                if (k > c) L->a[k] = pA ? ra[k] : rb[k];
+                if (k > c) L->a[k] = sel(pA, ra[k], rb[k]);
         }
         lds_sync();
         const cx<R> ip = L->a[c];
-        const cx<R> la = pA ? zero : ra[c] * ip;
-        const cx<R> lb = pB ? zero : rb[c] * ip;
+        const cx<R> la = sel(pA, zero, ra[c] * ip);
+        const cx<R> lb = sel(pB, zero, rb[c] * ip);
 #pragma unroll
         for (int k = 0; k < 17; ++k) {
             if (k > c) {
@@ -300,7 +300,7 @@ __device__ __forceinline__ void group_solve(int role, GroupLds<R> *L, const cx<d
         cx<R> acc = L->phat[pr][16];
 #pragma unroll
         for (int k = 0; k < 16; ++k) acc = fms(acc, L->phat[pr][k], L->a[k]);
-        x[2] = role < 7 ? cvt<double>(acc) : mk(0.0, 0.0);
+        x[2] = sel(role < 7, cvt<double>(acc), mk(0.0, 0.0));
     }
     lds_sync();
 }
@@ -334,7 +334,7 @@ __device__ __forceinline__ void draw_vec(int role, const PathState &P, uint64_t 
         const int yi = slot_index(role, k);
         double a, b;
         uniform_pair(base / 2 + (uint64_t)(yi < 0 ? 0 : yi), P.pid, P.seed, a, b);
-        out[k] = yi < 0 ? mk(0.0, 0.0) : mk((hi - lo) * a + lo, (hi - lo) * b + lo);
+        out[k] = sel(yi < 0, mk(0.0, 0.0), mk((hi - lo) * a + lo, (hi - lo) * b + lo));
     }
 }
 
@@ -344,8 +344,8 @@ __device__ __forceinline__ void switch_patch(int role, const cx<double> u[3], cx
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const bool is_t = (role == 7 && k == 1);
-        den = is_t ? den : fma(den, u[k], Y[k]);
-        num = is_t ? u[k] * Y[k] : num;
+        den = sel(is_t, den, fma(den, u[k], Y[k]));
+        num = sel(is_t, u[k] * Y[k], num);
     }
     den = grp_sum(den);
     num = grp_sum(num);
@@ -353,13 +353,30 @@ __device__ __forceinline__ void switch_patch(int role, const cx<double> u[3], cx
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const bool is_t = (role == 7 && k == 1);
-        Y[k] = is_t ? Y[k] : sc * Y[k];
+        Y[k] = sel(is_t, Y[k], sc * Y[k]);
     }
 }
 
 __device__ __forceinline__ double grp_norm(const cx<double> v[3]) {
     double a = abs2(v[0]) + abs2(v[1]) + abs2(v[2]);
     return sqrt(grp_sum(a));
+}
+
+// Tangent normalisation (htpysupport.cu:285-296).  Arclength: x * e^{-i arg x_t} / |x|, written
+// as x * conj(x_t)/(|x_t| |x|); t-step: x / x_t.  Then Im x_t = 0.
+__device__ __forceinline__ void normalize_tangent(int role, cx<double> x[3], bool tstep) {
+    const double nrm = grp_norm(x);
+    const cx<double> xt = grp_bcast(x[1], role, 7);
+    cx<double> sc;
+    if (!tstep) {
+        const double a = sqrt(abs2(xt));
+        sc = sel(a > 0.0, mk(xt.re / (a * nrm), -xt.im / (a * nrm)), mk(1.0 / nrm, 0.0));
+    } else {
+        sc = inv(xt);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) x[k] = sc * x[k];
+    if (role == 7) x[1].im = 0.0;
 }
 // t (real part of Y[22], held by lane 7 slot 1), delivered to the whole group
 __device__ __forceinline__ double grp_t(int role, const cx<double> Y[3]) { return grp_bcast(Y[1].re, role, 7); }
@@ -696,7 +713,7 @@ __device__ __forceinline__ void finish_path(int role, const Job &J, PathState &P
     const bool from_sharp = (P.phase == PH_SHARP);
     cx<double> Y[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) Y[k] = from_sharp ? P.K[0][k] : P.Yc[k];
+    for (int k = 0; k < 3; ++k) Y[k] = sel(from_sharp, P.K[0][k], P.Yc[k]);
     const int64_t path = P.path;
     if (J.yfinal) {
         cx<double> *yo = J.yfinal + path * 23;
@@ -751,12 +768,12 @@ __device__ __forceinline__ void finish_path(int role, const Job &J, PathState &P
         cx<double> o0 = zd[0], o1 = zd[1];
 #pragma unroll
         for (int i = 0; i < 14; ++i) {
-            o0 = (swap && i == 2 * role) ? cog[i] : o0;
-            o1 = (swap && i == 2 * role + 1) ? cog[i] : o1;
+            o0 = sel(swap && i == 2 * role, cog[i], o0);
+            o1 = sel(swap && i == 2 * role + 1, cog[i], o1);
         }
         out[2 * role] = o0;
         out[2 * role + 1] = o1;
-        out[14 + role] = swap ? Udown : zd[2];
+        out[14 + role] = sel(swap, Udown, zd[2]);
     }
     if (role == 0) {
         J.err[path] = P.code;
@@ -794,13 +811,13 @@ __device__ __forceinline__ void micro_step(int role, const TrackOpts &o, PathSta
             cx<double> acc = P.Yc[k];
 #pragma unroll
             for (int m = 0; m < 7; ++m) acc = acc + c[m] * P.K[m][k];
-            Ye[k] = newton ? P.K[0][k] : acc;
+            Ye[k] = sel(newton, P.K[0][k], acc);
             // w: Vc at init and stage 0, k_{stage-1} otherwise; e_t for Newton
             cx<double> wv = P.Vc[k];
 #pragma unroll
-            for (int m = 0; m < 7; ++m) wv = (rk && P.stage == m + 1) ? P.K[m][k] : wv;
+            for (int m = 0; m < 7; ++m) wv = sel(rk && P.stage == m + 1, P.K[m][k], wv);
             const bool is_t = (role == 7 && k == 1);
-            W[k] = newton ? (is_t ? mk(1.0, 0.0) : mk(0.0, 0.0)) : wv;
+            W[k] = sel(newton, mk(is_t ? 1.0 : 0.0, 0.0), wv);
         }
     }
     double tcur = 0.0;
@@ -838,19 +855,7 @@ __device__ __forceinline__ void micro_step(int role, const TrackOpts &o, PathSta
     group_solve<R>(role, L, Ye, P.u, W, newton, w_rhs, P.s6, P.d6, P.gamma, fN, x);
     if (!newton) {
         P.st_ivp++;
-        // tangent normalisation (htpysupport.cu:285-296)
-        const double nrm = grp_norm(x);
-        const cx<double> xt = grp_bcast(x[1], role, 7);
-        cx<double> sc;
-        if (!P.tstep || P.phase == PH_INIT) {
-            const double a = sqrt(abs2(xt));
-            sc = a > 0.0 ? mk(xt.re / (a * nrm), -xt.im / (a * nrm)) : mk(1.0 / nrm, 0.0);
-        } else {
-            sc = inv(xt);
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) x[k] = sc * x[k];
-        if (role == 7) x[1].im = 0.0;
+        normalize_tangent(role, x, P.tstep && P.phase != PH_INIT);
         if (P.phase == PH_INIT) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) P.Vc[k] = x[k];
@@ -861,7 +866,7 @@ __device__ __forceinline__ void micro_step(int role, const TrackOpts &o, PathSta
 #pragma unroll
         for (int m = 0; m < 8; ++m)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) P.K[m][k] = (P.stage == m) ? x[k] : P.K[m][k];
+            for (int k = 0; k < 3; ++k) P.K[m][k] = sel(P.stage == m, x[k], P.K[m][k]);
         P.stage++;
         if (P.stage == 8) finish_rk(role, o, P);
         return;
@@ -961,6 +966,130 @@ hipError_t launch_track(const Job &job, int kind, int block, int grid, hipStream
         if (mixed) hipLaunchKernelGGL((track_kernel<false, true>), dim3(grid), dim3(block), lds, stream, job);
         else hipLaunchKernelGGL((track_kernel<false, false>), dim3(grid), dim3(block), lds, stream, job);
     }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
+// Stage-level entry points for parity tests: one solve per path on explicit inputs, and the
+// start system alone.  They run the same device code as the tracker.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void load_slots(int role, const cx<double> *v23, cx<double> s[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int yi = slot_index(role, k);
+        s[k] = yi < 0 ? mk(0.0, 0.0) : v23[yi];
+    }
+}
+__device__ __forceinline__ void store_slots(int role, cx<double> *v23, const cx<double> s[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int yi = slot_index(role, k);
+        if (yi >= 0) v23[yi] = s[k];
+    }
+}
+
+// mode 0: arclength tangent (IVP tstepQ = false), 1: t-step tangent, 2: Newton step to tgoal
+// (returns the update), precision 0: FP64, 1: FP32 Jacobian and solve.
+template <typename R>
+__global__ __launch_bounds__(64) void debug_solve_kernel(DebugSolveArgs A) {
+    extern __shared__ unsigned char smem[];
+    const int role = threadIdx.x & 7;
+    const int grp = threadIdx.x >> 3;
+    GroupLds<R> *L = reinterpret_cast<GroupLds<R> *>(smem) + grp;
+    const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 3) + grp;
+    if (p >= A.n) return;
+    cx<double> Y[3], U[3], W[3], x[3], s6[6], d6[6];
+    load_slots(role, A.Y + p * 23, Y);
+    load_slots(role, A.u + p * 23, U);
+    load_slots(role, A.w + p * 23, W);
+    const int j = role < 7 ? role + 1 : 7;
+    const cx<double> *sp = A.spm + p * 30;
+    const int idx[6] = {0, j, 8, 8 + j, 15 + j, 22 + j};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        s6[q] = sp[idx[q]];
+        d6[q] = A.fpm[idx[q]] - s6[q];
+    }
+    const bool newton = A.mode == 2;
+    double w_rhs = 1.0;
+    cx<double> fN[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+    if (newton) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) W[k] = mk((role == 7 && k == 1) ? 1.0 : 0.0, 0.0);
+        w_rhs = A.tgoal[p] - grp_t(role, Y);
+        // FP64 residual values of this lane's equations
+        PoseEval<double> e;
+        const cx<double> *y = A.Y + p * 23;
+#pragma unroll
+        for (int i = 0; i < 14; ++i) e.L[i] = y[i];
+        e.Z0 = y[21];
+        e.t = y[22].re;
+        e.U = role < 7 ? Y[2] : mk(1.0, 0.0);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            e.s[q] = s6[q];
+            e.d[q] = d6[q];
+        }
+        e.gamma = A.gamma;
+        pose_values(e, fN[0], fN[1], fN[2]);
+    }
+    group_solve<R>(role, L, Y, U, W, newton, w_rhs, s6, d6, A.gamma, fN, x);
+    if (!newton) normalize_tangent(role, x, A.mode == 1);
+    store_slots(role, A.x + p * 23, x);
+}
+
+__global__ __launch_bounds__(64) void debug_start_kernel(DebugStartArgs A) {
+    const int role = threadIdx.x & 7;
+    const int grp = threadIdx.x >> 3;
+    const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 3) + grp;
+    if (p >= A.n) return;
+    PathState P;
+    P.seed = A.seed;
+    P.pid = A.pid0 + (uint64_t)p;
+    generate_start(role, A.o, P);
+    cx<double> *z = A.spt + p * 21;
+    cx<double> *q = A.spm + p * 30;
+    if (role < 7) {
+        const int j = role + 1;
+        z[2 * role] = P.Yc[0];
+        z[2 * role + 1] = P.Yc[1];
+        z[14 + role] = P.Yc[2];
+        q[j] = P.s6[1];
+        q[8 + j] = P.s6[3];
+        q[15 + j] = P.s6[4];
+        q[22 + j] = P.s6[5];
+    }
+    if (role == 0) {
+        q[0] = P.s6[0];
+        q[8] = P.s6[2];
+    }
+}
+
+hipError_t launch_debug_solve(const DebugSolveArgs &a, int precision, hipStream_t stream) {
+    const int block = 64;
+    const int grid = (int)((a.n + 7) / 8);
+    if (grid == 0) return hipSuccess;
+    if (precision) {
+        hipLaunchKernelGGL(debug_solve_kernel<float>, dim3(grid), dim3(block), 8 * sizeof(GroupLds<float>), stream, a);
+    } else {
+        hipLaunchKernelGGL(debug_solve_kernel<double>, dim3(grid), dim3(block), 8 * sizeof(GroupLds<double>), stream,
+                           a);
+    }
+    return hipGetLastError();
+}
+
+__global__ void reset_counter_kernel(unsigned long long *p) {
+    if (threadIdx.x == 0) *p = 0ull;
+}
+hipError_t launch_reset_counter(unsigned long long *p, hipStream_t stream) {
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_start(const DebugStartArgs &a, hipStream_t stream) {
+    const int grid = (int)((a.n + 7) / 8);
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(debug_start_kernel, dim3(grid), dim3(64), 0, stream, a);
     return hipGetLastError();
 }
 

@@ -1,0 +1,161 @@
+/*
+ * frg.h -- C ABI of the MI355X homotopy path tracker for Stephenson II finite root generation.
+ *
+ * Drop-in for the device side of mplecnik/Finite_Root_Generation: the reference's round body
+ *     InitRandStates<<<80,256>>>(d_randStates, seed);                               frg.cu:361
+ *     Homotopy<<<80,256>>>(d_randStates, d_fpm, d_fpt, d_errorCodes, ...);          frg.cu:374
+ *     cudaMemcpy(fpt, d_fpt, ...); cudaMemcpy(errorCodes, d_errorCodes, ...);        frg.cu:389-404
+ * (kernels at frg.cu:30-33 and frg.cu:35-124) becomes one call of frg_track().  Inputs and
+ * outputs keep the reference's layouts: the target parameters are the 30 complex values of
+ * Vect<nP> (frg.cu:149-178), each root is the 21 complex values of Vect<nV> (336 bytes, the
+ * record written to the fptColl file, frg.cu:472-476) and each error code is an int32 in 0..5
+ * (htpytrack.cu:85-216).  No HIP or torch types cross this boundary.
+ *
+ * Conventions: functions return FRG_OK (0) or a negative FRG_E* code; per-path outcomes are in
+ * the err[] output.  The caller owns every host buffer; a context owns its device memory and
+ * stream.  A context is used by one host thread at a time; separate contexts are independent.
+ */
+#ifndef FRG_H
+#define FRG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FRG_NV 21 /* variables, htpykinematics.cuh:8 */
+#define FRG_NP 30 /* parameters, htpykinematics.cuh:9 */
+#define FRG_NY 23 /* homogeneous path point [z, Z0, t] */
+
+enum {
+    FRG_OK = 0,
+    FRG_EINVAL = -1,      /* bad argument */
+    FRG_EHIP = -2,        /* HIP runtime error (device, launch, copy) */
+    FRG_ENOMEM = -3,      /* device or host allocation failed */
+    FRG_EUNSUPPORTED = -4 /* option combination not implemented */
+};
+
+enum { FRG_START_UNIFORM = 0, FRG_START_GAUSSIAN = 1 };          /* start draws, frgkinematics.cu:7-8 */
+enum { FRG_PREC_FP64 = 0, FRG_PREC_MIXED = 1 };                  /* arithmetic of the tracker */
+enum { FRG_SCHED_STATIC = 0, FRG_SCHED_PERSISTENT = 1 };          /* one path per group / work queue */
+enum { FRG_STEP_CLASSIC = 0, FRG_STEP_LEARNED = 1 };              /* step-size proposal */
+
+/* One complex number, bit-compatible with the reference's Cplx (classes.cuh:26-27). */
+typedef struct {
+    double re, im;
+} frg_c64;
+
+/* Tracker settings.  frg_default_opts() fills the reference's #defines (htpytrack.cu:9-31,
+ * frgkinematics.cu:7-8) and the FP64 / static / classic choices. */
+typedef struct {
+    double sinit, smax, smin, rk_tol;  /* htpytrack.cu:9-12 */
+    int32_t caution_steps, newt_its;   /* :13, :16 */
+    double newt_tol;                   /* :17 */
+    int32_t newt_attempts, sharp_its;  /* :19, :20 */
+    double sharp_tol;                  /* :21 */
+    int32_t sharp_attempts, newt_residual; /* :23, :18 (only 1 = |Ydiff| is supported) */
+    double sharp_trigger, nudge_trigger;   /* :24, :25 */
+    int32_t max_steps, max_ministeps;      /* :26, :27 */
+    double patch_lo, patch_hi;             /* :30-31 */
+    double start_lo, start_hi;             /* frgkinematics.cu:7-8 */
+    int32_t start_mode;                    /* FRG_START_* */
+    int32_t precision;                     /* FRG_PREC_* */
+    double fp64_band;                      /* mixed: FP64 once 1 - t < fp64_band */
+    int32_t schedule;                      /* FRG_SCHED_* */
+    int32_t block_size;                    /* threads per block (64..256, multiple of 64); 0 = tuned */
+    int32_t groups_per_cu;                 /* persistent grid: paths in flight per CU; 0 = occupancy */
+    int32_t step_control;                  /* FRG_STEP_* */
+} frg_opts;
+
+/* Per-call summary. */
+typedef struct {
+    int64_t n_paths;
+    int64_t code_count[8];  /* paths per error code 0..5 */
+    int64_t steps, rk_attempts, newton_calls, newton_its, tangent_solves;
+    double kernel_ms;       /* device time of the tracking launch (HIP events) */
+    double total_ms;        /* host wall time of the whole call, copies included */
+    int32_t block_size, grid_size, schedule, precision;
+} frg_stats;
+
+/* Opaque context: device, stream, device buffers, tuning cache. */
+typedef struct frg_ctx frg_ctx;
+
+/* Fills the defaults.  No reference counterpart (the reference uses #defines). */
+int frg_default_opts(frg_opts *opts);
+
+/* Creates a context on HIP device `device`.  Replaces the allocations and setup of
+ * frg.cu:138-236 (device properties, cudaMalloc of the round buffers). */
+int frg_create(frg_ctx **out, int device, const frg_opts *opts);
+
+/* Frees the context.  Replaces frg.cu:518-531. */
+void frg_destroy(frg_ctx *ctx);
+
+/* Replaces the opts of an existing context. */
+int frg_set_opts(frg_ctx *ctx, const frg_opts *opts);
+
+/* One round: n_paths paths, path i with RNG id first_path_id + i, random start systems
+ * (GenerateStart, frgkinematics.cu:13-136), gamma trick to the target fpm.  Host buffers:
+ * fpt (n x 21), err (n), path_stats (n x 8 int32: steps, RK attempts, Newton calls, Newton
+ * iterations, ministeps, patch re-draws, tangent solves, code; may be NULL), summary (may be
+ * NULL).  Replaces frg.cu:356-404 (InitRandStates + Homotopy + copies). */
+int frg_track(frg_ctx *ctx, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
+              int64_t n_paths, frg_c64 *fpt, int32_t *err, int32_t *path_stats, frg_stats *summary);
+
+/* The same with device pointers (no host copies); output buffers must hold n_paths records.
+ * `stream` is a hipStream_t (NULL: the context's stream).  The call returns after the work is
+ * enqueued; synchronise on the stream before reading. */
+int frg_track_device(frg_ctx *ctx, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
+                     int64_t n_paths, frg_c64 *d_fpt, int32_t *d_err, int32_t *d_path_stats, void *stream);
+
+/* Explicit start systems: spt (n x 21) solves F(spt; spm) = 0 for spm (n x 30).  TrackPath's
+ * patch draws for path i come from RNG id first_path_id + i, from draw 0.  yfinal (n x 23, may
+ * be NULL) receives the homogeneous end points before dehomogenisation.  Replaces the
+ * GenerateStart call in the Homotopy kernel (frg.cu:54) by caller-supplied data. */
+int frg_track_explicit(frg_ctx *ctx, const frg_c64 *fpm, frg_c64 gamma, const frg_c64 *spt, const frg_c64 *spm,
+                       uint64_t seed, uint64_t first_path_id, int64_t n_paths, frg_c64 *fpt, int32_t *err,
+                       int32_t *path_stats, frg_c64 *yfinal, frg_stats *summary);
+
+/* Several task instances in one launch: instance k has target fpm[30k..30k+29], gamma[k],
+ * seed seeds[k], RNG ids first_path_ids[k] + i, and paths_per_inst[k] paths; outputs are
+ * concatenated in instance order.  No reference counterpart (the reference tracks one target). */
+int frg_track_batch(frg_ctx *ctx, int32_t n_inst, const frg_c64 *fpm, const frg_c64 *gamma, const uint64_t *seeds,
+                    const uint64_t *first_path_ids, const int64_t *paths_per_inst, frg_c64 *fpt, int32_t *err,
+                    int32_t *path_stats, frg_stats *summary);
+
+/* Start systems alone (GenerateStart, frgkinematics.cu:13-136): spt (n x 21), spm (n x 30). */
+int frg_generate_starts(frg_ctx *ctx, uint64_t seed, uint64_t first_path_id, int64_t n, frg_c64 *spt, frg_c64 *spm);
+
+/* One linear solve per path on explicit inputs, for stage-level parity:
+ *   mode 0: IVP with tstepQ = false (htpysupport.cu:237-299), w = velocity patch row;
+ *   mode 1: IVP with tstepQ = true;
+ *   mode 2: one Newton update toward tgoal[i] (htpysupport.cu:430-480; w ignored).
+ * Y, u, w: n x 23; spm: n x 30; fpm: 30; x: n x 23 out.  precision FRG_PREC_FP64 or
+ * FRG_PREC_MIXED (FP32 Jacobian and solve, FP64 residual). */
+int frg_solve_step(frg_ctx *ctx, int32_t mode, int32_t precision, int64_t n, const frg_c64 *Y, const frg_c64 *u,
+                   const frg_c64 *w, const frg_c64 *spm, const frg_c64 *fpm, frg_c64 gamma, const double *tgoal,
+                   frg_c64 *x);
+
+/* Host post-processing (frgsupport.cu).  frg_sort_dedupe_merge replaces SortingDeletingMerging
+ * (frgsupport.cu:10-215) plus the device sort of frg.cu:432-435: merges the n new roots into
+ * the sorted collection coll (m roots), marks duplicates in err with code 6 (tolerance tol on
+ * every real, frgsupport.cu:7), writes the merged sorted collection to coll_out (room for
+ * m + n) and returns its length (negative on error).  frg_lambertw replaces LambertW
+ * (frgsupport.cu:249-272); frg_completeness gives the estimated fraction of roots collected,
+ * 1 + r W0(-e^{-1/r}/r) with r = successes/(successes + duplicates) (frg.cu:464-465). */
+int64_t frg_sort_dedupe_merge(const frg_c64 *coll, int64_t m, const frg_c64 *fpt, int32_t *err, int64_t n,
+                              double tol, frg_c64 *coll_out);
+double frg_lambertw(double z);
+double frg_completeness(int64_t successes, int64_t duplicates);
+
+/* Error text for a return code. */
+const char *frg_strerror(int code);
+
+/* ABI version (major * 10000 + minor * 100 + patch). */
+int frg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

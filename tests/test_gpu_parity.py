@@ -1,0 +1,145 @@
+"""GPU parity: the HIP tracker (through the C ABI) against the CPU oracle on identical inputs.
+
+Stage level: start systems, one tangent solve (IVP), one Newton update.  Path level: whole
+tracks from explicit and generated starts; per-path error codes must agree and code-0 roots
+must agree to 1e-8 (max |dx| over the 21 dehomogenised, cognate-canonicalised components,
+measured relative to max(1, |x|)).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from finite_root_generation_amd import capi, tasks
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT_TOL = 1e-8
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _opts_pair(start_mode=0, **kw):
+    oo = O.default_opts(start_mode=start_mode)
+    for k, v in kw.items():
+        setattr(oo, k, v)
+    return oo
+
+
+def rel_err(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b)))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_generate_starts(tracker, mode):
+    say(f"[gpu] generate_starts mode={mode} n=64")
+    tracker.set_opts(start_mode=mode)
+    try:
+        spt, spm = tracker.generate_starts(seed=11, n=64, first_path_id=5)
+    finally:
+        tracker.set_opts(start_mode=0)
+    oo = _opts_pair(mode)
+    for i in range(64):
+        z, q = O.generate_start(oo, 11, 5 + i)
+        assert rel_err(spt[i], z) < 1e-10, i
+        assert rel_err(spm[i], q) < 1e-10, i
+        # a start point solves its own start system
+        F = O.fsys(np.concatenate([spt[i], [1.0]]), spm[i])
+        assert np.max(np.abs(F)) < 1e-9 * (1 + np.max(np.abs(spt[i]))) ** 3
+
+
+def _stage_inputs(n, seed=3):
+    """Points near start systems with random patches, as the tracker sees them."""
+    rng = np.random.default_rng(seed)
+    oo = O.default_opts()
+    Y = np.zeros((n, 23), complex)
+    U = np.zeros((n, 23), complex)
+    W = np.zeros((n, 23), complex)
+    spm = np.zeros((n, 30), complex)
+    for i in range(n):
+        z, q = O.generate_start(oo, 99, i)
+        y = np.concatenate([z, [1.0, rng.uniform(0.0, 0.9)]])
+        y[:22] += 1e-3 * (rng.normal(size=22) + 1j * rng.normal(size=22))
+        u = rng.uniform(-2, 2, 23) + 1j * rng.uniform(-2, 2, 23)
+        Y[i] = O.switch_patch(u, y)
+        U[i] = u
+        W[i] = rng.uniform(-2, 2, 23) + 1j * rng.uniform(-2, 2, 23)
+        spm[i] = q
+    return Y, U, W, spm
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_solve_step_tangent(tracker, mode):
+    n = 64
+    Y, U, W, spm = _stage_inputs(n)
+    say(f"[gpu] solve_step tangent mode={mode} n={n}")
+    x = tracker.solve_step(mode, Y, U, W, spm, tasks.REF_FPM, tasks.REF_GAMMA)
+    for i in range(n):
+        ref = O.ivp(mode, Y[i], W[i], U[i], spm[i], tasks.REF_FPM, tasks.REF_GAMMA)
+        assert rel_err(x[i], ref) < 1e-9, (i, rel_err(x[i], ref))
+
+
+def test_solve_step_newton(tracker):
+    n = 64
+    Y, U, W, spm = _stage_inputs(n)
+    tg = np.real(Y[:, 22]) + 0.01
+    say(f"[gpu] solve_step newton n={n}")
+    x = tracker.solve_step(2, Y, U, None, spm, tasks.REF_FPM, tasks.REF_GAMMA, tgoal=tg)
+    for i in range(n):
+        YN, res, its = O.newton(tg[i], W[i], U[i], spm[i], tasks.REF_FPM, tasks.REF_GAMMA, 1, 0.0, 1, Y[i])
+        assert its == 1
+        assert rel_err(x[i], YN - Y[i]) < 1e-9, (i, rel_err(x[i], YN - Y[i]))
+
+
+def test_solve_step_fp32(tracker):
+    n = 64
+    Y, U, W, spm = _stage_inputs(n)
+    say(f"[gpu] solve_step fp32 tangent n={n}")
+    x = tracker.solve_step(0, Y, U, W, spm, tasks.REF_FPM, tasks.REF_GAMMA, precision=capi.PREC_MIXED)
+    errs = []
+    for i in range(n):
+        ref = O.ivp(0, Y[i], W[i], U[i], spm[i], tasks.REF_FPM, tasks.REF_GAMMA)
+        errs.append(rel_err(x[i], ref))
+    # FP32 Jacobian and solve: single-precision accuracy on well-conditioned systems
+    assert np.median(errs) < 1e-4, np.median(errs)
+
+
+def _compare_tracks(fpt_g, err_g, fpt_o, err_o, min_agree=1.0):
+    same_code = err_g == err_o
+    ok = tasks.finite_mask(fpt_o, err_o) & tasks.finite_mask(fpt_g, err_g)
+    d = np.max(np.abs(fpt_g - fpt_o) / np.maximum(1.0, np.abs(fpt_o)), axis=1)
+    root_ok = ~ok | (d <= ROOT_TOL)
+    agree = same_code & root_ok
+    frac = agree.mean()
+    say(f"    codes gpu {np.bincount(err_g, minlength=6)} oracle {np.bincount(err_o, minlength=6)}; "
+        f"agree {agree.sum()}/{len(agree)}; max root diff {np.max(d[ok]) if ok.any() else 0:.2e}")
+    assert frac >= min_agree, np.nonzero(~agree)[0][:20]
+
+
+def test_track_explicit_ref_target(tracker):
+    n = 256
+    oo = O.default_opts()
+    spt = np.zeros((n, 21), complex)
+    spm = np.zeros((n, 30), complex)
+    for i in range(n):
+        spt[i], spm[i] = O.generate_start(oo, 2024, i)
+    say(f"[gpu] track_explicit reference target n={n}")
+    fpt_g, err_g, st_g, yf_g, summ = tracker.track_explicit(tasks.REF_FPM, tasks.REF_GAMMA, spt, spm, seed=2024)
+    fpt_o, err_o, st_o, _ = O.homotopy_explicit(oo, tasks.REF_FPM, tasks.REF_GAMMA, spt, spm, 2024, 0)
+    _compare_tracks(fpt_g, err_g, fpt_o, err_o)
+
+
+def test_track_generated_seeded_task(tracker):
+    n = 512
+    fpm, gamma = tasks.seeded_task(0)
+    say(f"[gpu] track generated seeded task n={n}")
+    fpt_g, err_g, st_g, summ = tracker.track(fpm, gamma, seed=7, n=n)
+    oo = O.default_opts()
+    fpt_o, err_o, st_o, _ = O.homotopy(oo, fpm, gamma, 7, 0, n)
+    _compare_tracks(fpt_g, err_g, fpt_o, err_o)

@@ -1,0 +1,359 @@
+"""Pins the CPU oracle (oracle/frg_oracle.c) before it is trusted as the GPU checker.
+
+Every check comes from outside the oracle's own formulas: published known-answer vectors,
+numpy/LAPACK, finite differences, invariants of the problem (start points and their cognates
+are roots; patches are affine charts), order conditions, and the reference's own constants
+(tests/golden/reference_fixtures.json, extracted from frg.cu, htpytrack.cu, frgsupport.cu).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from finite_root_generation_amd import tasks
+from oracle import oracle as O
+from tests import stephenson_np as S
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = json.load(open(os.path.join(HERE, "golden", "reference_fixtures.json")))
+
+
+def cplx(pairs):
+    return np.array([complex(a, b) for a, b in pairs])
+
+
+# ---------------------------------------------------------------------------------------- RNG
+def test_philox_known_answers():
+    # Random123 known-answer vectors for Philox4x32-10 (kat_vectors, Salmon et al. SC'11)
+    assert [int(x) for x in O.philox([0, 0, 0, 0], [0, 0])] == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert [int(x) for x in O.philox([0xffffffff] * 4, [0xffffffff] * 2)] == \
+        [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert [int(x) for x in O.philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0])] \
+        == [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+def test_uniform_draws():
+    u = np.array([O.uniform(7, 3, k) for k in range(4000)])
+    assert np.all(u > 0) and np.all(u <= 1)
+    assert abs(u.mean() - 0.5) < 0.02 and abs(u.var() - 1 / 12) < 0.01
+    # draw 2b and 2b+1 are the two 64-bit halves of Philox block b
+    o = O.philox([5, 0, 3, 0], [7, 0])
+    w0 = (int(o[1]) << 32) | int(o[0])
+    assert O.uniform(7, 3, 10) == ((w0 >> 11) + 1) * 2.0 ** -53
+
+
+# ---------------------------------------------------------------------------- complex scalars
+def test_complex_division_and_sqrt():
+    rng = np.random.default_rng(0)
+    for _ in range(200):
+        a = complex(*rng.normal(size=2) * 10 ** rng.uniform(-3, 3, 2))
+        b = complex(*rng.normal(size=2) * 10 ** rng.uniform(-3, 3, 2))
+        assert abs(O.cdiv(a, b) - a / b) <= 4e-16 * abs(a / b)
+        x = complex(*rng.normal(size=2))
+        ref = np.sqrt(x)
+        # half-angle formulas: Im part carries ~eps |x| / (4 |Im sqrt x|) from the 1 - cos A cancellation
+        bound = 1e-15 * (1 + abs(x)) + 4e-16 * abs(x) / max(abs(ref.imag), 1e-300)
+        assert abs(O.csqrt(x) - ref) <= bound
+    # branch cut: the sign of the imaginary part (a signed zero included) selects the half plane
+    assert O.csqrt(complex(-4.0, 0.0)) == 2j
+    assert O.csqrt(complex(-4.0, -0.0)) == -2j
+
+
+# ------------------------------------------------------------------------------ the system
+@pytest.mark.parametrize("mode", [0, 1])
+def test_start_points_are_roots(mode):
+    oo = O.default_opts(start_mode=mode)
+    for pid in range(200):
+        z, q = O.generate_start(oo, 3, pid)
+        F = O.fsys(np.concatenate([z, [1.0]]), q)
+        assert np.max(np.abs(F)) <= 1e-11 * (1 + np.max(np.abs(z))) ** 3, pid
+        # start draws: Qc = 1/Q, free values within the draw box in uniform mode
+        assert np.allclose(q[23:30] * q[16:23], 1.0, rtol=1e-13)
+        if mode == 0:
+            assert np.all(np.abs(z[:14].real) <= 5) and np.all(np.abs(z[:14].imag) <= 5)
+
+
+def test_fsys_matches_independent_form():
+    rng = np.random.default_rng(1)
+    Z, q = S.random_point(rng, 50)
+    Fnp = S.fsys(Z, q)
+    for i in range(50):
+        Fo = O.fsys(Z[i], q[i])
+        assert np.max(np.abs(Fo - Fnp[i])) <= 1e-12 * (1 + np.max(np.abs(Fnp[i]))), i
+
+
+def test_fsys_homogeneous_degree_three():
+    rng = np.random.default_rng(2)
+    Z, q = S.random_point(rng, 5)
+    lam = complex(0.7, -1.3)
+    for i in range(5):
+        assert np.allclose(O.fsys(lam * Z[i], q[i]), lam ** 3 * O.fsys(Z[i], q[i]), rtol=1e-12, atol=1e-12)
+
+
+def _fd(f, x, h):
+    """Fourth-order central difference of a holomorphic vector function along each coordinate."""
+    cols = []
+    for k in range(x.size):
+        e = np.zeros(x.size, complex)
+        e[k] = h
+        cols.append((8 * (f(x + e) - f(x - e)) - (f(x + 2 * e) - f(x - 2 * e))) / (12 * h))
+    return np.array(cols).T
+
+
+def test_jvar_jpar_finite_differences():
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        Z, q = S.random_point(rng)
+        Z, q = Z[0], q[0]
+        J = O.jvar(Z, q)
+        Jfd = _fd(lambda z: O.fsys(z, q), Z, 1e-4)
+        assert np.max(np.abs(J - Jfd)) <= 1e-8 * np.max(np.abs(J))
+        Jp = O.jpar(Z, q)
+        Jpfd = _fd(lambda p: O.fsys(Z, p), q, 1e-4)
+        assert np.max(np.abs(Jp - Jpfd)) <= 1e-8 * np.max(np.abs(Jp))
+
+
+def test_jacobian_structure():
+    rng = np.random.default_rng(4)
+    Z, q = S.random_point(rng)
+    J = O.jvar(Z[0], q[0])
+    Jp = O.jpar(Z[0], q[0])
+    nz = np.abs(J) > 0
+    # per pose: I has 8 nonzero columns, II has 6, III has 16 (htpykinematics.cu:162-1124)
+    assert [int(nz[j].sum()) for j in range(7)] == [8] * 7
+    assert [int(nz[7 + j].sum()) for j in range(7)] == [6] * 7
+    assert [int(nz[14 + j].sum()) for j in range(7)] == [16] * 7
+    assert [int((np.abs(Jp[r]) > 0).sum()) for r in range(21)] == [6] * 7 + [4] * 7 + [6] * 7
+
+
+# ----------------------------------------------------------------------- homotopy support
+def test_q_and_dqdt():
+    rng = np.random.default_rng(5)
+    s = rng.normal(size=30) + 1j * rng.normal(size=30)
+    f = rng.normal(size=30) + 1j * rng.normal(size=30)
+    g = tasks.REF_GAMMA
+    assert np.allclose(O.q(0.0, s, f, g), s, rtol=1e-15)
+    assert np.allclose(O.q(1.0, s, f, g), f, rtol=1e-15)
+    for t in (0.1, 0.5, 0.93):
+        h = 1e-5
+        fd = (O.q(t + h, s, f, g) - O.q(t - h, s, f, g)) / (2 * h)
+        assert np.allclose(O.dqdt(t, s, f, g), fd, rtol=1e-8, atol=1e-10)
+        closed = g * (f - s) / (g * (1 - t) + t) ** 2
+        assert np.allclose(O.dqdt(t, s, f, g), closed, rtol=1e-14)
+
+
+def test_switch_patch_with_reference_patches():
+    u = cplx(FIX["u"])
+    u2 = cplx(FIX["u2"])
+    rng = np.random.default_rng(6)
+    Y = np.concatenate([rng.normal(size=22) + 1j * rng.normal(size=22), [0.3]])
+    for patch in (u, u2):
+        Yn = O.switch_patch(patch, Y)
+        assert abs(np.dot(patch, Yn) - 1) < 1e-13
+        assert Yn[22] == Y[22]
+        ratio = Yn[:22] / Y[:22]
+        assert np.allclose(ratio, ratio[0], rtol=1e-13)
+
+
+def test_linear_solve_vs_lapack():
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        M = rng.normal(size=(23, 23)) + 1j * rng.normal(size=(23, 23))
+        b = rng.normal(size=23) + 1j * rng.normal(size=23)
+        x = O.linear_solve(M.ravel(), b)
+        assert np.allclose(x, np.linalg.solve(M, b), rtol=1e-11, atol=1e-12)
+    # a zero leading pivot forces a row exchange
+    M = rng.normal(size=(23, 23)) + 0j
+    M[0, 0] = 0.0
+    b = rng.normal(size=23) + 0j
+    assert np.allclose(O.linear_solve(M.ravel(), b), np.linalg.solve(M, b), rtol=1e-11)
+
+
+def test_linear_solve_newton_ignores_last_row_and_pins_t():
+    rng = np.random.default_rng(8)
+    M = rng.normal(size=(23, 23)) + 1j * rng.normal(size=(23, 23))
+    M[22] = np.nan  # the velocity row is never read (htpysupport.cu:330,358)
+    b = rng.normal(size=22) + 1j * rng.normal(size=22)
+    x = O.linear_solve_newton(M.ravel(), b, 0.25)
+    assert x[22] == 0.25
+    assert np.allclose(M[:22, :22] @ x[:22] + M[:22, 22] * 0.25, b, rtol=1e-11, atol=1e-11)
+
+
+def _point_on_path(seed=9, t=0.4):
+    """A point on a homotopy path: track a start from t = 0 with tiny RK steps."""
+    oo = O.default_opts()
+    z, qs = O.generate_start(oo, seed, 0)
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(-2, 2, 23) + 1j * rng.uniform(-2, 2, 23)
+    Y = O.switch_patch(u, np.concatenate([z, [1.0, 0.0]]))
+    return Y, u, qs
+
+
+def test_ivp_tangent_properties():
+    Y, u, qs = _point_on_path()
+    rng = np.random.default_rng(10)
+    v = rng.normal(size=23) + 1j * rng.normal(size=23)
+    fpm, g = tasks.REF_FPM, tasks.REF_GAMMA
+    JH = O.JH(Y, u, qs, fpm, g)
+    d = O.ivp(0, Y, v, u, qs, fpm, g)
+    assert abs(np.linalg.norm(d) - 1) < 1e-13
+    assert d[22].imag == 0 and d[22].real >= 0
+    assert np.max(np.abs(JH @ d)) < 1e-10 * np.max(np.abs(JH))
+    # the unit tangent does not depend on the velocity patch row
+    d2 = O.ivp(0, Y, v * 1.7 + 0.3, u, qs, fpm, g)
+    assert np.allclose(d, d2, atol=1e-12)
+    dt = O.ivp(1, Y, v, u, qs, fpm, g)
+    assert abs(dt[22] - 1) < 1e-13
+    assert np.allclose(dt, d / d[22].real, rtol=1e-11, atol=1e-13)
+
+
+def test_rk65_orders():
+    """One RK6(5)8M step: the propagated solution has local error O(s^7), the embedded
+    estimate O(s^6) (rungekutta.cu); a wrong tableau entry drops the order."""
+    Y, u, qs = _point_on_path()
+    fpm, g = tasks.REF_FPM, tasks.REF_GAMMA
+    V = O.ivp(0, Y, np.ones(23, complex), u, qs, fpm, g)
+
+    def step(Y0, V0, s):
+        return O.rk65(s, 1e-3, 0, Y0, V0, u, qs, fpm, g)
+
+    def fine(s, n=64):
+        Yf, Vf = Y, V
+        for _ in range(n):
+            Yf, Vf, _, _ = step(Yf, Vf, s / n)
+        return Yf
+
+    errs, ests = [], []
+    for s in (0.04, 0.02):
+        Yn, Vn, err, _ = step(Y, V, s)
+        errs.append(np.linalg.norm(Yn - fine(s)))
+        ests.append(err)
+    order_sol = np.log2(errs[0] / errs[1])
+    order_est = np.log2(ests[0] / ests[1])
+    assert order_sol > 6.3, order_sol
+    assert 5.3 < order_est < 6.8, order_est
+
+
+def test_newton_converges_quadratically():
+    Y, u, qs = _point_on_path()
+    fpm, g = tasks.REF_FPM, tasks.REF_GAMMA
+    rng = np.random.default_rng(11)
+    Y0 = Y.copy()
+    Y0[:22] += 1e-6 * np.max(np.abs(Y)) * (rng.normal(size=22) + 1j * rng.normal(size=22))
+    V = np.ones(23, complex)
+    res = []
+    Yk = Y0
+    for _ in range(4):
+        Yk, r, its = O.newton(0.0, V, u, qs, fpm, g, 1, 0.0, 1, Yk)
+        res.append(r)
+    assert res[-1] < 1e-12
+    # e_{k+1} <= C e_k^2: a wrong Jacobian converges linearly and breaks this bound by orders
+    assert res[1] < 1e5 * res[0] ** 2
+    assert res[2] < 1e5 * res[1] ** 2 + 1e-13
+    assert np.max(np.abs(O.H(Yk, u, qs, fpm, g))) < 1e-12 * (1 + np.max(np.abs(Yk))) ** 3
+    assert Yk[22] == 0.0
+
+
+# ---------------------------------------------------------------------------- cognates
+def test_cognate_is_a_root():
+    """The 1/2 cognate of a root of F(.; q) is again a root (frgkinematics.cu:185-285)."""
+    oo = O.default_opts()
+    for pid in range(50):
+        z, q = O.generate_start(oo, 12, pid)
+        c = O.cognate_solns(z, q)
+        for k in range(2):
+            F = O.fsys(np.concatenate([c[k], [1.0]]), q)
+            assert np.max(np.abs(F)) < 1e-9 * (1 + np.max(np.abs(c[k]))) ** 3, (pid, k)
+        assert c[0][0].real <= c[1][0].real
+        # one of the two is the input itself
+        assert np.allclose(c[0], z) or np.allclose(c[1], z)
+
+
+# ---------------------------------------------------------------------------- whole paths
+def test_track_paths_reach_roots_of_target():
+    oo = O.default_opts()
+    fpm, g = tasks.REF_FPM, tasks.REF_GAMMA
+    fpt, err, st, yf = O.homotopy(oo, fpm, g, 123, 0, 32, nthreads=4, want_yfinal=True)
+    assert (err == 0).sum() >= 28
+    for i in np.nonzero(err == 0)[0]:
+        assert abs(yf[i, 22] - 1.0) == 0.0
+        F = O.fsys(yf[i, :22], fpm)
+        assert np.max(np.abs(F)) < 1e-10 * (1 + np.max(np.abs(yf[i, :22]))) ** 3
+        if tasks.finite_mask(fpt[i:i + 1], err[i:i + 1])[0]:
+            Fd = O.fsys(np.concatenate([fpt[i], [1.0]]), fpm)
+            assert np.max(np.abs(Fd)) < 1e-7 * (1 + np.max(np.abs(fpt[i]))) ** 3
+    assert np.all(st["ivp_calls"][err == 0] == 8 * st["rk_attempts"][err == 0] + 1)
+
+
+def test_track_is_deterministic_and_thread_independent():
+    oo = O.default_opts()
+    fpm, g = tasks.seeded_task(0)
+    a = O.homotopy(oo, fpm, g, 5, 100, 16, nthreads=1)
+    b = O.homotopy(oo, fpm, g, 5, 100, 16, nthreads=4)
+    assert np.array_equal(a[0].view(np.float64), b[0].view(np.float64), equal_nan=True)
+    assert np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("field,code", [("max_steps", 1), ("max_ministeps", 2)])
+def test_error_codes_forced(field, code):
+    """Tightening a limit forces the matching error code (htpytrack.cu:85, :95)."""
+    oo = O.default_opts(**{field: 3})
+    fpt, err, st, _ = O.homotopy(oo, tasks.REF_FPM, tasks.REF_GAMMA, 1, 0, 8)
+    assert np.all(err == code)
+
+
+def test_error_code_sharpening():
+    oo = O.default_opts(sharp_tol=0.0)  # unreachable: sharpening fails after its attempts
+    fpt, err, st, _ = O.homotopy(oo, tasks.REF_FPM, tasks.REF_GAMMA, 1, 0, 4)
+    assert np.all(err == 5)
+    assert np.all(st["redraws"] >= 3)
+
+
+# ------------------------------------------------------------------------- post-processing
+def test_lambertw():
+    from scipy.special import lambertw
+    # closed forms: W(-1/e) = -1, W(0) = 0, W(e) = 1 (special-cased at frgsupport.cu:254-256)
+    assert O.lambertw(-np.exp(-1.0)) == -1.0 and O.lambertw(0.0) == 0.0 and O.lambertw(np.e) == 1.0
+    for z in [-0.3678794, -0.3, -0.1, -1e-5, 1e-3, 0.5, 1.0, 10.0, 1e3]:
+        assert abs(O.lambertw(z) - lambertw(z).real) < 1e-12 * (1 + abs(lambertw(z).real)), z
+
+
+def _brute_dedupe(coll, fpt, err, tol):
+    err = err.copy()
+    for t in range(len(fpt)):
+        if err[t] != 0:
+            continue
+        if any(np.all(np.abs(fpt[t].view(float) - c.view(float)) <= tol) for c in coll):
+            err[t] = 6
+            continue
+        for u in range(t):
+            if err[u] == 0 and np.all(np.abs(fpt[t].view(float) - fpt[u].view(float)) <= tol):
+                err[t] = 6
+                break
+    return err
+
+
+def test_sort_dedupe_merge_vs_brute_force():
+    rng = np.random.default_rng(13)
+    base = rng.normal(size=(40, 21)) + 1j * rng.normal(size=(40, 21))
+    coll = base[:15][np.argsort(base[:15, 0].real)]
+    picks = rng.integers(0, 40, 60)
+    fpt = base[picks] + 1e-8 * rng.normal(size=(60, 21))
+    err = rng.choice([0, 0, 0, 0, 3], 60).astype(np.int32)
+    out, err2 = O.sort_dedupe_merge(coll, fpt, err, 1e-6)
+    assert np.array_equal(err2, _brute_dedupe(coll, fpt, err, 1e-6))
+    assert np.all(np.diff(out[:, 0].real) >= 0)
+    assert len(out) == len(coll) + int((err2 == 0).sum())
+
+
+def test_reference_fixtures_consistent():
+    assert np.array_equal(cplx(FIX["fpm"]), tasks.REF_FPM)
+    assert complex(*FIX["gamma"]) == tasks.REF_GAMMA
+    oo = O.default_opts()
+    d = FIX["defines"]
+    assert oo.smin == d["htpytrack.cu:smin"] and oo.rk_tol == d["htpytrack.cu:RKtol"]
+    assert oo.newt_tol == d["htpytrack.cu:NewtTol"] and oo.sharp_tol == d["htpytrack.cu:SharpTol"]
+    assert oo.max_steps == d["htpytrack.cu:maxSteps"] and oo.max_ministeps == d["htpytrack.cu:maxMinisteps"]
+    assert (oo.patch_lo, oo.patch_hi) == (d["htpytrack.cu:LowerRand"], d["htpytrack.cu:UpperRand"])
+    assert (oo.start_lo, oo.start_hi) == (d["frgkinematics.cu:LowerRand"], d["frgkinematics.cu:UpperRand"])
