@@ -1,0 +1,169 @@
+"""Benchmark: tracked paths/s on one MI355X, chosen tracker vs plain FP64, at equal recall.
+
+Workload (one "step" = one pass over one batch): the Stephenson II finite-root-generation round
+at full size, 1,048,576 paths; target task from seed 1 (finite_root_generation_amd.tasks
+.seeded_task), start systems drawn by the tracker's Philox stream from seed 2.  Inputs are the
+30 target parameters and the seed; outputs (roots, codes, per-path counters) stay in HBM, so
+the timed region is the tracking launch alone.  The same batch is run by
+  * plain:  FP64, one path per 8-lane group, no refill (the reference's launch shape);
+  * chosen: FP32 Jacobian/solve away from t = 1, persistent grid with a device work queue.
+value = chosen paths/s.  One JSON line on stdout.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--paths N] [--cpu-sample N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "tracked paths/s on 1 MI355X, chosen vs plain FP64, at equal finite-root recall"
+P64_TFLOPS = 78.6    # MI355X FP64 vector/matrix peak
+P32_TFLOPS = 157.3   # MI355X FP32 vector peak (published)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_variant(capi, torch, name, opts_kw, fpm, gamma, n, steps, warmup, seed):
+    t = capi.Tracker(0, **opts_kw)
+    dev = torch.device("cuda:0")
+    fpt = torch.empty((n, 21), dtype=torch.complex128, device=dev)
+    err = torch.empty((n,), dtype=torch.int32, device=dev)
+    st = torch.empty((n, 8), dtype=torch.int32, device=dev)
+    for w in range(warmup):
+        log(f"[bench] {name}: warmup {w + 1}/{warmup}")
+        t.track_device(fpm, gamma, seed, n, fpt.data_ptr(), err.data_ptr(), st.data_ptr())
+        torch.cuda.synchronize()
+    kern = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        t.track_device(fpm, gamma, seed, n, fpt.data_ptr(), err.data_ptr(), st.data_ptr())
+        kern.append(t.last_kernel_ms())
+        log(f"[bench] {name}: step {k + 1}/{steps} kernel {kern[-1]:.1f} ms")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = {
+        "fpt": fpt.cpu().numpy(), "err": err.cpu().numpy(),
+        "stats": st.cpu().numpy().view(capi.PATH_STATS_DTYPE).reshape(n),
+        "ms_per_step": 1e3 * wall / steps, "kernel_ms": float(np.mean(kern)),
+    }
+    t.close()
+    return out
+
+
+def flops_split(capi, stats, model):
+    """(FP64 flops, FP32 flops) of the tracked paths: FP32 solves counted separately."""
+    solves = stats["tangent_solves"].astype(np.float64) + stats["newton_its"].astype(np.float64)
+    f32_share = np.where(solves > 0, stats["fp32_solves"] / np.maximum(solves, 1), 0.0)
+    per_solve = (stats["tangent_solves"] * model["tangent_solve"] + stats["newton_its"] * model["newton_update"])
+    f32 = float((per_solve * f32_share).sum())
+    f64 = float((per_solve * (1 - f32_share)).sum() + stats["rk_attempts"].astype(np.float64).sum()
+                * model["rk_combination"])
+    return f64, f32
+
+
+def roofline(capi, res, model):
+    f64, f32 = flops_split(capi, res["stats"], model)
+    sec = res["kernel_ms"] * 1e-3
+    tmin = f64 / (P64_TFLOPS * 1e12) + f32 / (P32_TFLOPS * 1e12)
+    achieved = (f64 + f32) / sec / 1e12
+    peak = (f64 + f32) / tmin / 1e12 if tmin > 0 else P64_TFLOPS
+    return {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 2), "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4), "traffic": None,
+            "flops_per_launch": {"fp64": f64, "fp32": f32}}
+
+
+def compare(tasks, a, b):
+    """Per-path agreement of finite roots (rel 1e-8) and recall of b's finite roots by a."""
+    fa = tasks.finite_mask(a["fpt"], a["err"])
+    fb = tasks.finite_mask(b["fpt"], b["err"])
+    both = fa & fb
+    d = np.full(len(fa), np.inf)
+    d[both] = np.max(np.abs(a["fpt"][both] - b["fpt"][both]) / np.maximum(1.0, np.abs(b["fpt"][both])), axis=1)
+    match = both & (d <= 1e-8)
+    return {"finite_a": int(fa.sum()), "finite_b": int(fb.sum()), "per_path_match": int(match.sum()),
+            "recall": float(match.sum() / max(1, fb.sum()))}
+
+
+def cpu_baseline(fpm, gamma, seed, n_sample):
+    from oracle import oracle as O
+    cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    cores = max(1, min(cores, 16))
+    oo = O.default_opts()
+    t0 = time.perf_counter()
+    O.homotopy(oo, fpm, gamma, seed, 0, n_sample, nthreads=cores, fast=True)
+    dt = time.perf_counter() - t0
+    return {"value": round(n_sample / dt, 2), "unit": "paths/s", "cores": cores, "kind": "port",
+            "sample": f"first {n_sample} paths of the same workload (same task, seeds and RNG ids), "
+                      f"CPU oracle -O3, OpenMP {cores} threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--paths", type=int, default=1 << 20)
+    ap.add_argument("--cpu-sample", type=int, default=0, help="paths for the CPU baseline (0: ~10 s of work)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
+                                   "and not launched", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+
+    from finite_root_generation_amd import capi, tasks
+
+    n = args.paths
+    fpm, gamma = tasks.seeded_task(1)
+    seed = 2
+    model = capi.flop_model()
+    plain = run_variant(capi, torch, "plain", dict(precision=capi.PREC_FP64, schedule=capi.SCHED_STATIC),
+                        fpm, gamma, n, args.steps, args.warmup, seed)
+    chosen = run_variant(capi, torch, "chosen", dict(precision=capi.PREC_MIXED, schedule=capi.SCHED_PERSISTENT),
+                         fpm, gamma, n, args.steps, args.warmup, seed)
+    value = n / (chosen["ms_per_step"] * 1e-3)
+    plain_value = n / (plain["ms_per_step"] * 1e-3)
+    cmp = compare(tasks, chosen, plain)
+    lengths = plain["stats"]["steps"]
+    hist, edges = np.histogram(lengths, bins=[0, 25, 50, 75, 100, 150, 200, 300, 500, 1000, 2001])
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "paths/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(chosen["ms_per_step"], 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
+        "data": "synthetic: seeded generic task (seed 1), Philox start systems (seed 2)",
+        "config": {"workload": "stephenson2_frg_full_size", "paths": n, "task_seed": 1, "start_seed": 2,
+                   "chosen": "mixed FP32/FP64 + persistent work queue", "plain": "FP64, static"},
+        "roofline": roofline(capi, chosen, model),
+        "plain": {"value": round(plain_value, 1), "ms_per_step": round(plain["ms_per_step"], 2),
+                  "roofline": roofline(capi, plain, model)},
+        "speedup_vs_plain": round(value / plain_value, 3),
+        "finite_roots": {"chosen": cmp["finite_a"], "plain": cmp["finite_b"]},
+        "failures": {"chosen": np.bincount(chosen["err"], minlength=6).tolist(),
+                     "plain": np.bincount(plain["err"], minlength=6).tolist()},
+        "recall_vs_plain": round(cmp["recall"], 6),
+        "path_length_hist": {"edges": edges.tolist(), "counts": hist.tolist()},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+        ns = args.cpu_sample or max(64, 250 * min(16, cores))
+        log(f"[bench] cpu baseline: {ns} paths")
+        line["cpu_baseline"] = cpu_baseline(fpm, gamma, seed, ns)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -27,13 +27,14 @@ ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "out of 
           -4: "unsupported option"}
 
 PATH_STATS_FIELDS = ["steps", "rk_attempts", "newton_calls", "newton_its", "ministeps", "redraws",
-                     "tangent_solves", "code"]
+                     "tangent_solves", "fp32_solves"]
 PATH_STATS_DTYPE = np.dtype([(f, np.int32) for f in PATH_STATS_FIELDS])
 
 # every entry point declared in include/frg.h
 EXPORTS = ["frg_default_opts", "frg_create", "frg_destroy", "frg_set_opts", "frg_track", "frg_track_device",
            "frg_track_explicit", "frg_track_batch", "frg_generate_starts", "frg_solve_step",
-           "frg_sort_dedupe_merge", "frg_lambertw", "frg_completeness", "frg_strerror", "frg_version"]
+           "frg_sort_dedupe_merge", "frg_lambertw", "frg_completeness", "frg_strerror", "frg_version",
+           "frg_last_kernel_ms", "frg_flops"]
 
 
 class C64(ct.Structure):
@@ -81,6 +82,13 @@ class Stats(ct.Structure):
         return d
 
 
+class FlopModel(ct.Structure):
+    """Mirror of frg_flop_model."""
+
+    _fields_ = [("pose_rows", ct.c_double), ("pose_values", ct.c_double), ("tangent_solve", ct.c_double),
+                ("newton_update", ct.c_double), ("rk_combination", ct.c_double)]
+
+
 class FrgError(RuntimeError):
     pass
 
@@ -117,6 +125,8 @@ def lib() -> ct.CDLL:
         L.frg_strerror.argtypes = [ct.c_int]
         L.frg_strerror.restype = ct.c_char_p
         L.frg_version.restype = ct.c_int
+        L.frg_last_kernel_ms.argtypes = [P, ct.POINTER(ct.c_double)]
+        L.frg_flops.argtypes = [ct.POINTER(FlopModel)]
         _lib = L
     return _lib
 
@@ -234,6 +244,11 @@ class Tracker:
                                       ct.c_void_p(d_err), ct.c_void_p(d_stats) if d_stats else None,
                                       ct.c_void_p(stream) if stream else None), "frg_track_device")
 
+    def last_kernel_ms(self) -> float:
+        v = ct.c_double()
+        _check(lib().frg_last_kernel_ms(self._h, ct.byref(v)), "frg_last_kernel_ms")
+        return v.value
+
     def generate_starts(self, seed: int, n: int, first_path_id: int = 0):
         spt = np.zeros((n, NV), np.complex128)
         spm = np.zeros((n, NP), np.complex128)
@@ -263,6 +278,20 @@ def sort_dedupe_merge(coll, fpt, err, tol: float = 1e-6):
     if k < 0:
         raise FrgError("frg_sort_dedupe_merge: invalid argument")
     return out[:k], err
+
+
+def flop_model() -> dict:
+    m = FlopModel()
+    _check(lib().frg_flops(ct.byref(m)), "frg_flops")
+    return {f: getattr(m, f) for f, _ in m._fields_}
+
+
+def path_flops(stats, model=None) -> float:
+    """Algorithmic FLOPs of the tracked paths from their per-path counters."""
+    m = model or flop_model()
+    return float(stats["tangent_solves"].astype(np.float64).sum() * m["tangent_solve"]
+                 + stats["newton_its"].astype(np.float64).sum() * m["newton_update"]
+                 + stats["rk_attempts"].astype(np.float64).sum() * m["rk_combination"])
 
 
 def lambertw(z: float) -> float:

@@ -395,6 +395,16 @@ int frg_track_batch(frg_ctx *c, int32_t n_inst, const frg_c64 *fpm, const frg_c6
     return run_host(c, job, fpt, err, path_stats, nullptr, summary, t0);
 }
 
+int frg_last_kernel_ms(frg_ctx *c, double *ms) {
+    if (!c || !ms) return FRG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float v = 0.f;
+    HIPCHK(hipEventElapsedTime(&v, c->ev0, c->ev1));
+    *ms = v;
+    return FRG_OK;
+}
+
 int frg_generate_starts(frg_ctx *c, uint64_t seed, uint64_t first_path_id, int64_t n, frg_c64 *spt, frg_c64 *spm) {
     if (!c || n < 0 || (n > 0 && (!spt || !spm))) return FRG_EINVAL;
     if (n == 0) return FRG_OK;

@@ -29,46 +29,46 @@ __host__ __device__ __forceinline__ cx<R> mk(R a, R b) {
     return z;
 }
 template <typename R>
-__device__ __forceinline__ cx<R> operator+(cx<R> a, cx<R> b) { return mk(a.re + b.re, a.im + b.im); }
+__host__ __device__ __forceinline__ cx<R> operator+(cx<R> a, cx<R> b) { return mk(a.re + b.re, a.im + b.im); }
 template <typename R>
-__device__ __forceinline__ cx<R> operator-(cx<R> a, cx<R> b) { return mk(a.re - b.re, a.im - b.im); }
+__host__ __device__ __forceinline__ cx<R> operator-(cx<R> a, cx<R> b) { return mk(a.re - b.re, a.im - b.im); }
 template <typename R>
-__device__ __forceinline__ cx<R> operator-(cx<R> a) { return mk(-a.re, -a.im); }
+__host__ __device__ __forceinline__ cx<R> operator-(cx<R> a) { return mk(-a.re, -a.im); }
 template <typename R>
-__device__ __forceinline__ cx<R> operator*(cx<R> a, cx<R> b) {
+__host__ __device__ __forceinline__ cx<R> operator*(cx<R> a, cx<R> b) {
     return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
 }
 template <typename R>
-__device__ __forceinline__ cx<R> operator*(R s, cx<R> a) { return mk(s * a.re, s * a.im); }
+__host__ __device__ __forceinline__ cx<R> operator*(R s, cx<R> a) { return mk(s * a.re, s * a.im); }
 template <typename R>
-__device__ __forceinline__ R abs2(cx<R> a) { return a.re * a.re + a.im * a.im; }
+__host__ __device__ __forceinline__ R abs2(cx<R> a) { return a.re * a.re + a.im * a.im; }
 template <typename R>
-__device__ __forceinline__ cx<R> conj(cx<R> a) { return mk(a.re, -a.im); }
+__host__ __device__ __forceinline__ cx<R> conj(cx<R> a) { return mk(a.re, -a.im); }
 // 1/a = conj(a)/|a|^2 (one real reciprocal); values here stay far from over/underflow.
 template <typename R>
-__device__ __forceinline__ cx<R> inv(cx<R> a) {
+__host__ __device__ __forceinline__ cx<R> inv(cx<R> a) {
     R r = R(1) / abs2(a);
     return mk(a.re * r, -a.im * r);
 }
 template <typename R>
-__device__ __forceinline__ cx<R> cdiv(cx<R> a, cx<R> b) { return a * inv(b); }
+__host__ __device__ __forceinline__ cx<R> cdiv(cx<R> a, cx<R> b) { return a * inv(b); }
 // acc - a*b
 template <typename R>
-__device__ __forceinline__ cx<R> fms(cx<R> acc, cx<R> a, cx<R> b) {
+__host__ __device__ __forceinline__ cx<R> fms(cx<R> acc, cx<R> a, cx<R> b) {
     return mk(acc.re - a.re * b.re + a.im * b.im, acc.im - a.re * b.im - a.im * b.re);
 }
 // acc + a*b
 template <typename R>
-__device__ __forceinline__ cx<R> fma(cx<R> acc, cx<R> a, cx<R> b) {
+__host__ __device__ __forceinline__ cx<R> fma(cx<R> acc, cx<R> a, cx<R> b) {
     return mk(acc.re + a.re * b.re - a.im * b.im, acc.im + a.re * b.im + a.im * b.re);
 }
 template <typename R, typename S>
-__device__ __forceinline__ cx<R> cvt(cx<S> a) { return mk(R(a.re), R(a.im)); }
+__host__ __device__ __forceinline__ cx<R> cvt(cx<S> a) { return mk(R(a.re), R(a.im)); }
 template <typename R>
-__device__ __forceinline__ cx<R> sel(bool c, cx<R> a, cx<R> b) { return mk(c ? a.re : b.re, c ? a.im : b.im); }
+__host__ __device__ __forceinline__ cx<R> sel(bool c, cx<R> a, cx<R> b) { return mk(c ? a.re : b.re, c ? a.im : b.im); }
 
 // Principal square root by the half-angle formulas of classes.cuh:39-45.
-__device__ __forceinline__ cx<double> csqrt_ref(cx<double> x) {
+__host__ __device__ __forceinline__ cx<double> csqrt_ref(cx<double> x) {
     double r = sqrt(x.re * x.re + x.im * x.im);
     double cosA = x.re / r;
     cx<double> o = mk(sqrt(r * (cosA + 1.0) / 2.0), sqrt(r * (1.0 - cosA) / 2.0));

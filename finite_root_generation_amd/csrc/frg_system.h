@@ -48,7 +48,7 @@ struct PoseEval {
 // q(t) = s + t/(gamma(1-t)+t) (f - s) and dq/dt = gamma/(gamma(1-t)+t)^2 (f - s): the reference's
 // q and dqdt (htpysupport.cu:37-62) rearranged.  Returns kappa = gamma/den^2.
 template <typename R>
-__device__ __forceinline__ cx<R> params_at(const PoseEval<R> &e, cx<R> q[6]) {
+__host__ __device__ __forceinline__ cx<R> params_at(const PoseEval<R> &e, cx<R> q[6]) {
     cx<R> den = mk(e.gamma.re + e.t * (R(1) - e.gamma.re), e.gamma.im * (R(1) - e.t));
     cx<R> id = inv(den);
     cx<R> tau = e.t * id;
@@ -59,7 +59,7 @@ __device__ __forceinline__ cx<R> params_at(const PoseEval<R> &e, cx<R> q[6]) {
 
 // F values only (the residual for Newton, used in FP64 when the Jacobian is FP32).
 template <typename R>
-__device__ __forceinline__ void pose_values(const PoseEval<R> &e, cx<R> &fI, cx<R> &fII, cx<R> &fIII) {
+__host__ __device__ __forceinline__ void pose_values(const PoseEval<R> &e, cx<R> &fI, cx<R> &fII, cx<R> &fIII) {
     cx<R> q[6];
     params_at(e, q);
     const cx<R> P0 = q[0], Pj = q[1], Pc0 = q[2], Pcj = q[3], Q = q[4], Qc = q[5];
@@ -96,7 +96,7 @@ __device__ __forceinline__ void pose_values(const PoseEval<R> &e, cx<R> &fI, cx<
 
 // Full rows of the three equations of one pose.
 template <typename R>
-__device__ __forceinline__ void pose_rows(const PoseEval<R> &e, Row<R> &rI, Row<R> &rII, Row<R> &rIII) {
+__host__ __device__ __forceinline__ void pose_rows(const PoseEval<R> &e, Row<R> &rI, Row<R> &rII, Row<R> &rIII) {
     cx<R> q[6];
     const cx<R> kap = params_at(e, q);
     const cx<R> P0 = q[0], Pj = q[1], Pc0 = q[2], Pcj = q[3], Q = q[4], Qc = q[5];

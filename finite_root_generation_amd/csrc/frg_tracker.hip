@@ -323,7 +323,7 @@ struct PathState {
     uint64_t draw, pid, seed;
     int64_t path;
     int inst;
-    int st_rk, st_newton_calls, st_newton_its, st_redraws, st_ivp;
+    int st_rk, st_newton_calls, st_newton_its, st_redraws, st_ivp, st_fp32;
 };
 
 // Patch draw (htpytrack.cu:55-58,147-150): Y index i uses draws base + 2i, base + 2i + 1.
@@ -528,6 +528,7 @@ __device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, 
     P.tgoal = 0.0;
     P.st_rk = P.st_newton_calls = P.st_newton_its = P.st_redraws = 0;
     P.st_ivp = 0;
+    P.st_fp32 = 0;
 #pragma unroll
     for (int m = 0; m < 8; ++m)
 #pragma unroll
@@ -786,7 +787,7 @@ __device__ __forceinline__ void finish_path(int role, const Job &J, PathState &P
             st[4] = P.ministep;
             st[5] = P.st_redraws;
             st[6] = P.st_ivp;
-            st[7] = P.code;
+            st[7] = P.st_fp32;
         }
     }
 }
@@ -920,8 +921,12 @@ __global__ __launch_bounds__(256) void track_kernel(Job J) {
     for (;;) {
         if (kMixed) {
             const bool need64 = needs_fp64(role, J.o, P);
-            if (__any(need64)) micro_step<double>(role, J.o, P, L);
-            else micro_step<float>(role, J.o, P, reinterpret_cast<GroupLds<float> *>(L));
+            if (__any(need64)) {
+                micro_step<double>(role, J.o, P, L);
+            } else {
+                P.st_fp32++;
+                micro_step<float>(role, J.o, P, reinterpret_cast<GroupLds<float> *>(L));
+            }
         } else {
             micro_step<double>(role, J.o, P, L);
         }

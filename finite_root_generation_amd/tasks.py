@@ -33,15 +33,32 @@ REF_FPM = np.array([
 ], dtype=np.complex128)
 
 
-def seeded_task(seed: int):
-    """Returns (fpm[30], gamma) of the seeded real task."""
+def _disk(rng, n):
+    return np.sqrt(rng.random(n)) * np.exp(2j * np.pi * rng.random(n))
+
+
+def seeded_task(seed: int, real: bool = False):
+    """Returns (fpm[30], gamma) of a seeded task.
+
+    Generic (default): the precision points P0..P7 and their isotropic partners Pc0..Pc7 are
+    drawn independently, uniform in the unit disk, and Q_j, Qc_j are standard complex Gaussian,
+    so the target is a generic complex parameter point like the reference's own fpm
+    (frg.cu:149-178, where Pc != conj P and Qc != 1/Q).
+    real=True: a physical task, Pc = conj(P), Q_j on the unit circle and Qc = conj(Q); this is a
+    special parameter point at which many paths end at singular roots.
+    gamma is on the unit circle in both cases.
+    """
     rng = np.random.default_rng(seed)
-    r = np.sqrt(rng.random(8))
-    th = 2 * np.pi * rng.random(8)
-    P = r * np.exp(1j * th)
-    Q = np.exp(2j * np.pi * rng.random(7))
+    P = _disk(rng, 8)
+    if real:
+        Q = np.exp(2j * np.pi * rng.random(7))
+        Pc, Qc = np.conj(P), np.conj(Q)
+    else:
+        Pc = _disk(rng, 8)
+        Q = (rng.normal(size=7) + 1j * rng.normal(size=7)) / np.sqrt(2)
+        Qc = (rng.normal(size=7) + 1j * rng.normal(size=7)) / np.sqrt(2)
     gamma = complex(np.exp(2j * np.pi * rng.random()))
-    fpm = np.concatenate([P, np.conj(P), Q, np.conj(Q)]).astype(np.complex128)
+    fpm = np.concatenate([P, Pc, Q, Qc]).astype(np.complex128)
     return fpm, gamma
 
 

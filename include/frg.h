@@ -98,7 +98,7 @@ int frg_set_opts(frg_ctx *ctx, const frg_opts *opts);
 /* One round: n_paths paths, path i with RNG id first_path_id + i, random start systems
  * (GenerateStart, frgkinematics.cu:13-136), gamma trick to the target fpm.  Host buffers:
  * fpt (n x 21), err (n), path_stats (n x 8 int32: steps, RK attempts, Newton calls, Newton
- * iterations, ministeps, patch re-draws, tangent solves, code; may be NULL), summary (may be
+ * iterations, ministeps, patch re-draws, tangent solves, solves done in FP32; may be NULL), summary (may be
  * NULL).  Replaces frg.cu:356-404 (InitRandStates + Homotopy + copies). */
 int frg_track(frg_ctx *ctx, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
               int64_t n_paths, frg_c64 *fpt, int32_t *err, int32_t *path_stats, frg_stats *summary);
@@ -148,6 +148,23 @@ int64_t frg_sort_dedupe_merge(const frg_c64 *coll, int64_t m, const frg_c64 *fpt
                               double tol, frg_c64 *coll_out);
 double frg_lambertw(double z);
 double frg_completeness(int64_t successes, int64_t duplicates);
+
+/* Device time (ms, HIP events) of the last tracking launch of the context; waits for it to end.
+ * No reference counterpart (the reference times RNG init + kernel + copies, frg.cu:356-426). */
+int frg_last_kernel_ms(frg_ctx *ctx, double *ms);
+
+/* Algorithmic floating-point operations per unit of work of this implementation's formulation
+ * (real +, -, *, / count 1; counted once per path, not per lane).  A tracked path costs
+ *   tangent_solves * tangent_solve + newton_its * newton_update + rk_attempts * rk_combination
+ * (counters from path_stats).  No reference counterpart. */
+typedef struct {
+    double pose_rows;       /* one pose: values, Jacobian rows and dF/dt of its three equations */
+    double pose_values;     /* one pose: the three residuals only */
+    double tangent_solve;   /* one tangent (IVP) solve, normalisation included */
+    double newton_update;   /* one Newton iteration */
+    double rk_combination;  /* stage points, error estimate and solution of one RK attempt */
+} frg_flop_model;
+int frg_flops(frg_flop_model *out);
 
 /* Error text for a return code. */
 const char *frg_strerror(int code);

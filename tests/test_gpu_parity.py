@@ -143,3 +143,95 @@ def test_track_generated_seeded_task(tracker):
     oo = O.default_opts()
     fpt_o, err_o, st_o, _ = O.homotopy(oo, fpm, gamma, 7, 0, n)
     _compare_tracks(fpt_g, err_g, fpt_o, err_o)
+
+
+def test_track_real_task_singular_endpoints(tracker):
+    """A real task (Pc = conj P) sends many paths to singular roots, where sharpening fails
+    (code 5) and rounding decides the end point; agreement is high but not total there."""
+    n = 256
+    fpm, gamma = tasks.seeded_task(0, real=True)
+    say(f"[gpu] track real task n={n}")
+    fpt_g, err_g, st_g, summ = tracker.track(fpm, gamma, seed=7, n=n)
+    fpt_o, err_o, st_o, _ = O.homotopy(O.default_opts(), fpm, gamma, 7, 0, n)
+    _compare_tracks(fpt_g, err_g, fpt_o, err_o, min_agree=0.95)
+
+
+def test_persistent_equals_static(tracker):
+    """A path's arithmetic does not depend on which group tracks it: bit-identical results."""
+    n = 1000
+    fpm, gamma = tasks.seeded_task(3)
+    say(f"[gpu] static vs persistent n={n}")
+    a = tracker.track(fpm, gamma, seed=9, n=n)
+    tracker.set_opts(schedule=capi.SCHED_PERSISTENT)
+    try:
+        b = tracker.track(fpm, gamma, seed=9, n=n)
+    finally:
+        tracker.set_opts(schedule=capi.SCHED_STATIC)
+    assert np.array_equal(a[1], b[1])
+    assert np.array_equal(a[0].view(np.float64), b[0].view(np.float64), equal_nan=True)
+    assert np.array_equal(a[2], b[2])
+
+
+def test_mixed_precision_recall(tracker):
+    """FP32 Jacobian/solve away from t = 1, FP64 residuals, end game and sharpening: the same
+    finite roots at 1e-8 as FP64 for nearly every path, and per-root error against the oracle."""
+    n = 512
+    fpm, gamma = tasks.seeded_task(1)
+    say(f"[gpu] mixed precision n={n}")
+    a = tracker.track(fpm, gamma, seed=2, n=n)
+    tracker.set_opts(precision=capi.PREC_MIXED)
+    try:
+        b = tracker.track(fpm, gamma, seed=2, n=n)
+    finally:
+        tracker.set_opts(precision=capi.PREC_FP64)
+    fa, fb = tasks.finite_mask(a[0], a[1]), tasks.finite_mask(b[0], b[1])
+    both = fa & fb
+    d = np.max(np.abs(b[0][both] - a[0][both]) / np.maximum(1.0, np.abs(a[0][both])), axis=1)
+    recall = (d <= 1e-8).sum() / max(1, fa.sum())
+    say(f"    fp64 finite {fa.sum()} mixed finite {fb.sum()} recall {recall:.4f}; "
+        f"fp32 solves {b[2]['fp32_solves'].sum()} of {(b[2]['tangent_solves'] + b[2]['newton_its']).sum()}")
+    assert b[2]["fp32_solves"].sum() > 0
+    assert recall >= 0.97
+    # the matched roots are FP64-sharpened: error against FP64 far below the tolerance
+    assert np.median(d[d <= 1e-8]) < 1e-11
+
+
+def test_batch_equals_single_instances(tracker):
+    fpms, gammas = zip(*[tasks.seeded_task(s) for s in (4, 5, 6)])
+    counts = [100, 37, 64]
+    seeds = [11, 12, 13]
+    pids = [0, 1000, 5]
+    say("[gpu] batch of 3 instances")
+    fpt, err, st, summ = tracker.track_batch(np.array(fpms), np.array(gammas), seeds, pids, counts)
+    off = 0
+    for k in range(3):
+        f1, e1, s1, _ = tracker.track(fpms[k], gammas[k], seed=seeds[k], n=counts[k], first_path_id=pids[k])
+        assert np.array_equal(e1, err[off:off + counts[k]])
+        assert np.array_equal(f1.view(np.float64), fpt[off:off + counts[k]].view(np.float64), equal_nan=True)
+        off += counts[k]
+
+
+def test_empty_and_tiny_launches(tracker):
+    fpm, gamma = tasks.seeded_task(0)
+    say("[gpu] n=0 and n=1")
+    fpt, err, st, summ = tracker.track(fpm, gamma, seed=1, n=0)
+    assert fpt.shape == (0, 21)
+    fpt, err, st, summ = tracker.track(fpm, gamma, seed=1, n=1)
+    fo, eo, so, _ = O.homotopy(O.default_opts(), fpm, gamma, 1, 0, 1)
+    assert err[0] == eo[0]
+
+
+def test_yfinal_lands_on_target(tracker):
+    n = 64
+    oo = O.default_opts()
+    spt = np.zeros((n, 21), complex)
+    spm = np.zeros((n, 30), complex)
+    for i in range(n):
+        spt[i], spm[i] = O.generate_start(oo, 77, i)
+    say(f"[gpu] yfinal n={n}")
+    fpt, err, st, yf, summ = tracker.track_explicit(tasks.REF_FPM, tasks.REF_GAMMA, spt, spm, seed=77,
+                                                    want_yfinal=True)
+    for i in np.nonzero(err == 0)[0]:
+        assert yf[i, 22] == 1.0
+        F = O.fsys(yf[i, :22], tasks.REF_FPM)
+        assert np.max(np.abs(F)) < 1e-10 * (1 + np.max(np.abs(yf[i, :22]))) ** 3
