@@ -295,6 +295,20 @@ def test_track_is_deterministic_and_thread_independent():
     assert np.array_equal(a[1], b[1])
 
 
+def test_rounding_sensitivity_classifier():
+    """The GPU parity tests accept a per-path disagreement only on paths the oracle itself
+    sends elsewhere under 1e-14 perturbations.  Path 79 of this workload is such a path; the
+    first paths are not."""
+    from tests.test_gpu_parity import rounding_sensitive
+    oo = O.default_opts()
+    fpm, g = tasks.seeded_task(0)
+    z, q = O.generate_start(oo, 7, 79)
+    assert rounding_sensitive(oo, fpm, g, z, q, 7, 79)
+    for pid in range(3):
+        z, q = O.generate_start(oo, 7, pid)
+        assert not rounding_sensitive(oo, fpm, g, z, q, 7, pid)
+
+
 @pytest.mark.parametrize("field,code", [("max_steps", 1), ("max_ministeps", 2)])
 def test_error_codes_forced(field, code):
     """Tightening a limit forces the matching error code (htpytrack.cu:85, :95)."""
