@@ -44,10 +44,26 @@ template <typename R>
 __host__ __device__ __forceinline__ R abs2(cx<R> a) { return a.re * a.re + a.im * a.im; }
 template <typename R>
 __host__ __device__ __forceinline__ cx<R> conj(cx<R> a) { return mk(a.re, -a.im); }
+// Real reciprocal.  On the device: the hardware estimate (v_rcp_f64 / v_rcp_f32) refined by
+// Newton-Raphson steps r <- r + r(1 - x r), two for FP64 and one for FP32, instead of the full
+// IEEE division sequence; the operands here are finite, nonzero and far from the exponent
+// limits.  Generic R (used by the host-side operation counter): 1/x.
+template <typename R>
+__host__ __device__ __forceinline__ R recip(R x) { return R(1) / x; }
+__device__ __forceinline__ double recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+    return r;
+}
+__device__ __forceinline__ float recip(float x) {
+    float r = __builtin_amdgcn_rcpf(x);
+    return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
+}
 // 1/a = conj(a)/|a|^2 (one real reciprocal); values here stay far from over/underflow.
 template <typename R>
 __host__ __device__ __forceinline__ cx<R> inv(cx<R> a) {
-    R r = R(1) / abs2(a);
+    R r = recip(abs2(a));
     return mk(a.re * r, -a.im * r);
 }
 template <typename R>

@@ -94,22 +94,50 @@ __host__ __device__ __forceinline__ void pose_values(const PoseEval<R> &e, cx<R>
     }
 }
 
-// Full rows of the three equations of one pose.
+// Quantities shared by the three equations of one pose.
 template <typename R>
-__host__ __device__ __forceinline__ void pose_rows(const PoseEval<R> &e, Row<R> &rI, Row<R> &rII, Row<R> &rIII) {
-    cx<R> q[6];
-    const cx<R> kap = params_at(e, q);
-    const cx<R> P0 = q[0], Pj = q[1], Pc0 = q[2], Pcj = q[3], Q = q[4], Qc = q[5];
-    const cx<R> dP0 = e.d[0], dPj = e.d[1], dPc0 = e.d[2], dPcj = e.d[3], dQ = e.d[4], dQc = e.d[5];
-    const cx<R> Z0 = e.Z0, U = e.U;
-    const cx<R> iU = inv(U);
-    const cx<R> zu = Z0 * iU;
-    const cx<R> one = mk(R(1), R(0)), zero = mk(R(0), R(0));
-    const cx<R> A = e.L[0], B = e.L[1], C = e.L[2], D = e.L[3], F = e.L[4], G = e.L[5], H = e.L[6];
-    const cx<R> Ac = e.L[7], Bc = e.L[8], Cc = e.L[9], Dc = e.L[10], Fc = e.L[11], Gc = e.L[12], Hc = e.L[13];
-    const cx<R> CA = C - A, CcAc = Cc - Ac;
-    const cx<R> P0Z = P0 * Z0, PjZ = Pj * Z0, Pc0Z = Pc0 * Z0, PcjZ = Pcj * Z0;
+struct PoseCommon {
+    cx<R> P0, Pj, Pc0, Pcj, Q, Qc, dP0, dPj, dPc0, dPcj, dQ, dQc, Z0, U, iU, zu, one, zero;
+    cx<R> A, B, C, D, F, G, H, Ac, Bc, Cc, Dc, Fc, Gc, Hc, CA, CcAc, P0Z, PjZ, Pc0Z, PcjZ, kap;
+};
 
+template <typename R>
+__host__ __device__ __forceinline__ void pose_common(const PoseEval<R> &e, PoseCommon<R> &c) {
+    cx<R> q[6];
+    c.kap = params_at(e, q);
+    c.P0 = q[0], c.Pj = q[1], c.Pc0 = q[2], c.Pcj = q[3], c.Q = q[4], c.Qc = q[5];
+    c.dP0 = e.d[0], c.dPj = e.d[1], c.dPc0 = e.d[2], c.dPcj = e.d[3], c.dQ = e.d[4], c.dQc = e.d[5];
+    c.Z0 = e.Z0, c.U = e.U;
+    c.iU = inv(e.U);
+    c.zu = e.Z0 * c.iU;
+    c.one = mk(R(1), R(0)), c.zero = mk(R(0), R(0));
+    c.A = e.L[0], c.B = e.L[1], c.C = e.L[2], c.D = e.L[3], c.F = e.L[4], c.G = e.L[5], c.H = e.L[6];
+    c.Ac = e.L[7], c.Bc = e.L[8], c.Cc = e.L[9], c.Dc = e.L[10], c.Fc = e.L[11], c.Gc = e.L[12], c.Hc = e.L[13];
+    c.CA = c.C - c.A, c.CcAc = c.Cc - c.Ac;
+    c.P0Z = c.P0 * c.Z0, c.PjZ = c.Pj * c.Z0, c.Pc0Z = c.Pc0 * c.Z0, c.PcjZ = c.Pcj * c.Z0;
+}
+
+#define FRG_POSE_UNPACK(c)                                                                            \
+    const cx<R> &P0 = c.P0, &Pj = c.Pj, &Pc0 = c.Pc0, &Pcj = c.Pcj, &Q = c.Q, &Qc = c.Qc;             \
+    const cx<R> &dP0 = c.dP0, &dPj = c.dPj, &dPc0 = c.dPc0, &dPcj = c.dPcj, &dQ = c.dQ, &dQc = c.dQc; \
+    const cx<R> &Z0 = c.Z0, &U = c.U, &iU = c.iU, &zu = c.zu, &one = c.one, &zero = c.zero;           \
+    const cx<R> &A = c.A, &B = c.B, &C = c.C, &D = c.D, &F = c.F, &G = c.G, &H = c.H;                 \
+    const cx<R> &Ac = c.Ac, &Bc = c.Bc, &Cc = c.Cc, &Dc = c.Dc, &Fc = c.Fc, &Gc = c.Gc, &Hc = c.Hc;   \
+    const cx<R> &CA = c.CA, &CcAc = c.CcAc, &P0Z = c.P0Z, &PjZ = c.PjZ, &Pc0Z = c.Pc0Z;               \
+    const cx<R> &PcjZ = c.PcjZ, &kap = c.kap;                                                          \
+    (void)P0, (void)Pj, (void)Pc0, (void)Pcj, (void)Q, (void)Qc, (void)dP0, (void)dPj, (void)dPc0;      \
+    (void)dPcj, (void)dQ, (void)dQc, (void)iU, (void)one, (void)zero, (void)A, (void)B, (void)C, (void)D; \
+    (void)F, (void)G, (void)H, (void)Ac, (void)Bc, (void)Cc, (void)Dc, (void)Fc, (void)Gc, (void)Hc;    \
+    (void)CA, (void)CcAc, (void)P0Z, (void)PjZ, (void)Pc0Z, (void)PcjZ, (void)kap
+
+// Row of equation I_j, II_j or III_j (EQ = 0, 1, 2) of one pose.
+template <int EQ, typename R>
+__host__ __device__ __forceinline__ void pose_row(const PoseEval<R> &e, const PoseCommon<R> &c, Row<R> &row) {
+    FRG_POSE_UNPACK(c);
+    (void)e;
+    Row<R> &rI = row, &rII = row, &rIII = row;
+    (void)rI, (void)rII, (void)rIII;
+    if (EQ == 0)
     // ---------------- I_j ----------------
     {
         const cx<R> QQ = Q * Qc, omQQ = one - QQ, CACc = CA * CcAc;
@@ -137,7 +165,7 @@ __host__ __device__ __forceinline__ void pose_rows(const PoseEval<R> &e, Row<R> 
         rI.g[15] = kap * (Z0 * Kd - Wd * Wc - W * Wcd);
     }
     // ---------------- II_j ----------------
-    {
+    if (EQ == 1) {
         const cx<R> X = PjZ - B, Y = P0Z - F, Xc = PcjZ - Bc, Yc = Pc0Z - Fc;
         const cx<R> BF = B - F, BFc = Bc - Fc;
         const cx<R> K = BF * BFc;
@@ -156,7 +184,7 @@ __host__ __device__ __forceinline__ void pose_rows(const PoseEval<R> &e, Row<R> 
         rII.g[15] = kap * (-(Wd * Wc) - W * Wcd);
     }
     // ---------------- III_j ----------------
-    {
+    if (EQ == 2) {
         const cx<R> iFB = inv(F - B), iHC = inv(H - C), iFBc = inv(Fc - Bc), iHCc = inv(Hc - Cc);
         const cx<R> r1 = (D - B) * iFB, r2 = (G - C) * iHC;
         const cx<R> r1c = (Dc - Bc) * iFBc, r2c = (Gc - Cc) * iHCc;
@@ -211,6 +239,16 @@ __host__ __device__ __forceinline__ void pose_rows(const PoseEval<R> &e, Row<R> 
         const cx<R> Wd = Z0 * Xd - U * Yd, Wcd = Xcd - zu * Ycd;
         rIII.g[15] = kap * (-(Wd * Wc) - W * Wcd);
     }
+}
+
+// Full rows of the three equations of one pose.
+template <typename R>
+__host__ __device__ __forceinline__ void pose_rows(const PoseEval<R> &e, Row<R> &rI, Row<R> &rII, Row<R> &rIII) {
+    PoseCommon<R> c;
+    pose_common(e, c);
+    pose_row<0>(e, c, rI);
+    pose_row<1>(e, c, rII);
+    pose_row<2>(e, c, rIII);
 }
 
 }  // namespace frg

@@ -68,6 +68,17 @@ __device__ __forceinline__ double rk_a(int i, int m) {
     v = (i == 7 && m == 5) ? 376225. / 454272. : v;
     return v;
 }
+// the same coefficients as a table, read with the (per-group) stage index
+__constant__ double kA[8][7] = {
+    {0, 0, 0, 0, 0, 0, 0},
+    {1. / 10., 0, 0, 0, 0, 0, 0},
+    {-2. / 81., 20. / 81., 0, 0, 0, 0, 0},
+    {615. / 1372., -270. / 343., 1053. / 1372., 0, 0, 0, 0},
+    {3243. / 5500., -54. / 55., 50949. / 71500., 4998. / 17875., 0, 0, 0},
+    {-26492. / 37125., 72. / 55., 2808. / 23375., -24206. / 37125., 338. / 459., 0, 0},
+    {5561. / 2376., -35. / 11., -24117. / 31603., 899983. / 200772., -5225. / 1836., 3925. / 4056., 0},
+    {465467. / 266112., -2945. / 1232., -5610201. / 14158144., 10513573. / 3212352., -424325. / 205632.,
+     376225. / 454272., 0}};
 // error (rungekutta.cu:99-105) and propagated solution (:115-122) weights; k2 enters neither
 __constant__ double kE[8] = {-13. / 2400., 0., 19683. / 618800., -2401. / 31200., 65. / 816., -15. / 416.,
                               -521. / 5600., 1. / 10.};
@@ -94,6 +105,23 @@ __device__ __forceinline__ int slot_index(int role, int k) {
     return k == 2 ? -1 : 21 + k;
 }
 
+// Row scaling by 1/max(|re|,|im|) over the row (htpysupport.cu:260-277, :446-471); the Newton
+// right-hand side -F is scaled by the same factor.  kOwnF: F comes with the row (FP64);
+// otherwise it is the FP64 residual fext.
+template <typename R, bool kOwnF>
+__device__ __forceinline__ void scale_row(const Row<R> &r, bool newton, cx<double> fext, cx<R> g[16], cx<R> &au,
+                                          cx<R> &rhs) {
+    R m = fmax(fabs(r.au.re), fabs(r.au.im));
+#pragma unroll
+    for (int k = 0; k < 16; ++k) m = fmax(m, fmax(fabs(r.g[k].re), fabs(r.g[k].im)));
+    const R s = recip(m);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) g[k] = s * r.g[k];
+    au = s * r.au;
+    const cx<double> f = sel(kOwnF, cvt<double>(r.f), fext);
+    rhs = sel(newton, cvt<R>(mk(-f.re * double(s), -f.im * double(s))), mk(R(0), R(0)));
+}
+
 // ------------------------------------------------------------------------------------------
 // The group solve.  Inputs are this lane's slots of the evaluation point Ye, the patch u and the
 // last row w; `newton` selects the Newton right-hand side (-H scaled; w rhs = dt) over the
@@ -103,7 +131,7 @@ __device__ __forceinline__ int slot_index(int role, int k) {
 template <typename R>
 __device__ __forceinline__ void group_solve(int role, GroupLds<R> *L, const cx<double> Ye[3], const cx<double> uu[3],
                                             const cx<double> ww[3], bool newton, double w_rhs,
-                                            const cx<double> s6[6], const cx<double> d6[6], cx<double> gamma,
+                                            const cx<double> *par, cx<double> gamma,
                                             const cx<double> fnewt[3], cx<double> x[3]) {
     constexpr bool kOwnF = sizeof(R) == sizeof(double);  // FP64 rows carry F themselves
     const cx<R> zero = mk(R(0), R(0));
@@ -135,68 +163,55 @@ __device__ __forceinline__ void group_solve(int role, GroupLds<R> *L, const cx<d
     e.U = sel(role < 7, cvt<R>(Ye[2]), mk(R(1), R(0)));
 #pragma unroll
     for (int p = 0; p < 6; ++p) {
-        e.s[p] = cvt<R>(s6[p]);
-        e.d[p] = cvt<R>(d6[p]);
+        e.s[p] = cvt<R>(par[p]);
+        e.d[p] = cvt<R>(par[6 + p]);
     }
     e.gamma = cvt<R>(gamma);
 
-    // -------- 2. pose rows, row scaling, in-lane elimination of U_j --------
-    Row<R> r0, r1, r2;
-    pose_rows(e, r0, r1, r2);
-    cx<R> rhs0 = zero, rhs1 = zero, rhs2 = zero;
-    {
-        R m0 = fmax(fabs(r0.au.re), fabs(r0.au.im)), m1 = fmax(fabs(r1.au.re), fabs(r1.au.im)),
-          m2 = fmax(fabs(r2.au.re), fabs(r2.au.im));
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            m0 = fmax(m0, fmax(fabs(r0.g[k].re), fabs(r0.g[k].im)));
-            m1 = fmax(m1, fmax(fabs(r1.g[k].re), fabs(r1.g[k].im)));
-            m2 = fmax(m2, fmax(fabs(r2.g[k].re), fabs(r2.g[k].im)));
-        }
-        const R s0 = R(1) / m0, s1 = R(1) / m1, s2 = R(1) / m2;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            r0.g[k] = s0 * r0.g[k];
-            r1.g[k] = s1 * r1.g[k];
-            r2.g[k] = s2 * r2.g[k];
-        }
-        r0.au = s0 * r0.au;
-        r1.au = s1 * r1.au;
-        r2.au = s2 * r2.au;
-        if (newton) {
-            const cx<double> f0 = kOwnF ? cvt<double>(r0.f) : fnewt[0];
-            const cx<double> f1 = kOwnF ? cvt<double>(r1.f) : fnewt[1];
-            const cx<double> f2 = kOwnF ? cvt<double>(r2.f) : fnewt[2];
-            rhs0 = cvt<R>(mk(-f0.re * double(s0), -f0.im * double(s0)));
-            rhs1 = cvt<R>(mk(-f1.re * double(s1), -f1.im * double(s1)));
-            rhs2 = cvt<R>(mk(-f2.re * double(s2), -f2.im * double(s2)));
-        }
-    }
-    // pivot among the three rows (first maximum wins, as in LinearSolve)
-    const R k0 = abs2(r0.au), k1 = abs2(r1.au), k2 = abs2(r2.au);
-    int pi = (k1 > k0) ? 1 : 0;
-    const R kb = (k1 > k0) ? k1 : k0;
-    pi = (k2 > kb) ? 2 : pi;
-    const bool p0 = pi == 0, p1 = pi == 1;
-    cx<R> pg[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) pg[k] = sel(p0, r0.g[k], sel(p1, r1.g[k], r2.g[k]));
-    const cx<R> pau = sel(p0, r0.au, sel(p1, r1.au, r2.au));
-    const cx<R> prhs = sel(p0, rhs0, sel(p1, rhs1, rhs2));
-    const cx<R> ipau = inv(pau);
-    // the two non-pivot rows, in order
+    // -------- 2. pose rows one at a time: row scaling, pairwise pivoting on |dF/dU_j| --------
+    // Rows I_j, II_j, III_j are produced in turn.  p holds the pivot candidate (largest |dF/dU_j|
+    // so far, the first one on ties, as LinearSolve's search); each other row is reduced by it,
+    // with multipliers of modulus <= 1.  The final p is the row of largest |dF/dU_j|.
+    PoseCommon<R> pc;
+    pose_common(e, pc);
+    cx<R> pg[16], pau, prhs;
     cx<R> ra[17], rb[17];
     {
-        const cx<R> ma = sel(p0, r1.au, r0.au) * ipau;  // row a = first non-pivot
-        const cx<R> mb = sel(pi == 2, r1.au, r2.au) * ipau;  // row b = second non-pivot
+        Row<R> r;
+        pose_row<0>(e, pc, r);
+        scale_row<R, kOwnF>(r, newton, fnewt[0], pg, pau, prhs);
+    }
+#pragma unroll
+    for (int eq = 1; eq < 3; ++eq) {
+        Row<R> r;
+        if (eq == 1) pose_row<1>(e, pc, r);
+        else pose_row<2>(e, pc, r);
+        cx<R> g[16], au, rhs;
+        scale_row<R, kOwnF>(r, newton, fnewt[eq], g, au, rhs);
+        const bool swap = abs2(au) > abs2(pau);
+        // (p, r) <- swap ? (r, p) : (p, r), then r <- r - (r_U / p_U) p
+        cx<R> red[17];
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            ra[k] = fms(sel(p0, r1.g[k], r0.g[k]), ma, pg[k]);
-            rb[k] = fms(sel(pi == 2, r1.g[k], r2.g[k]), mb, pg[k]);
+            const cx<R> pk = sel(swap, g[k], pg[k]);
+            red[k] = sel(swap, pg[k], g[k]);
+            pg[k] = pk;
         }
-        ra[16] = fms(sel(p0, rhs1, rhs0), ma, prhs);
-        rb[16] = fms(sel(pi == 2, rhs1, rhs2), mb, prhs);
+        red[16] = sel(swap, prhs, rhs);
+        prhs = sel(swap, rhs, prhs);
+        const cx<R> rU = sel(swap, pau, au);
+        pau = sel(swap, au, pau);
+        const cx<R> m = rU * inv(pau);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) red[k] = fms(red[k], m, pg[k]);
+        red[16] = fms(red[16], m, prhs);
+#pragma unroll
+        for (int k = 0; k < 17; ++k) {
+            if (eq == 1) ra[k] = red[k];
+            else rb[k] = red[k];
+        }
     }
+    const cx<R> ipau = inv(pau);
     // normalised pivot row -> LDS; U-entries of u (scaled) and w -> LDS
     const cx<R> uU = cvt<R>(usc * uu[2]);
     const cx<R> wU = cvt<R>(ww[2]);
@@ -259,21 +274,25 @@ __device__ __forceinline__ void group_solve(int role, GroupLds<R> *L, const cx<d
         const bool bBetter = kbv > ka;
         const int gid = grp_argmax(bBetter ? kbv : ka, bBetter ? idB : idA);
         const bool pA = gid == idA, pB = gid == idB;
+        // one reciprocal per step: of whichever of this lane's rows is the pivot
+        const cx<R> ipiv = inv(sel(pB, rb[c], ra[c]));
         if (pA) {
             usedA = true;
             pcA = c;
-            invA = inv(ra[c]);
+            invA = ipiv;
+            L->a[c] = ipiv;
+#pragma unroll
+            for (int k = 0; k < 17; ++k)
+                if (k > c) L->a[k] = ra[k];
         }
         if (pB) {
             usedB = true;
             pcB = c;
-            invB = inv(rb[c]);
-        }
-        if (pA || pB) {
-            L->a[c] = sel(pA, invA, invB);
+            invB = ipiv;
+            L->a[c] = ipiv;
 #pragma unroll
             for (int k = 0; k < 17; ++k)
-                if (k > c) L->a[k] = sel(pA, ra[k], rb[k]);
+                if (k > c) L->a[k] = rb[k];
         }
         lds_sync();
         const cx<R> ip = L->a[c];
@@ -310,11 +329,19 @@ __device__ __forceinline__ void group_solve(int role, GroupLds<R> *L, const cx<d
 // ------------------------------------------------------------------------------------------
 enum Phase : int { PH_INIT = 0, PH_RK = 1, PH_NEWTON = 2, PH_SHARP = 3 };
 
+// Per-group LDS: the solve scratch plus state that is read once per solve and so lives in LDS
+// rather than in VGPRs (the kernel's occupancy is register-bound, LDS has room).
+struct GroupSlot {
+    GroupLds<double> lds;   // also used as GroupLds<float> by FP32 solves
+    cx<double> par[8][12];  // per lane: start values s6[0..5] and target - start d6[0..5] of its pose
+    cx<double> vc[8][3];    // per lane: its slots of the current velocity Vc
+};
+
 struct PathState {
     // vectors (this lane's slots)
-    cx<double> Yc[3], Vc[3], u[3], K[8][3];  // K[0] doubles as YN, K[1] as Yn after an RK attempt
-    // pose parameters of this lane (start values, target - start)
-    cx<double> s6[6], d6[6], f6[6];
+    cx<double> Yc[3], u[3], K[8][3];  // K[0] doubles as YN, K[1] as Yn after an RK attempt
+    cx<double> *par;                 // this lane's row of GroupSlot::par (s6, d6)
+    cx<double> *vc;                  // this lane's row of GroupSlot::vc
     cx<double> gamma;
     // scalars (identical in all lanes of the group)
     double s, err, scale, residual, tgoal;
@@ -431,12 +458,12 @@ __device__ __forceinline__ void generate_start(int role, const TrackOpts &o, Pat
     const cx<double> S = SUc * U;
     const cx<double> Pj = B + (S * (F - B) + U * (P0 - F));
     const cx<double> Pcj = Bc + (inv(S) * (Fc - Bc) + inv(U) * (Pc0 - Fc));
-    P.s6[0] = P0;
-    P.s6[1] = Pj;
-    P.s6[2] = Pc0;
-    P.s6[3] = Pcj;
-    P.s6[4] = Q;
-    P.s6[5] = Qc;
+    P.par[0] = P0;
+    P.par[1] = Pj;
+    P.par[2] = Pc0;
+    P.par[3] = Pcj;
+    P.par[4] = Q;
+    P.par[5] = Qc;
     if (role < 7) {
         cx<double> a0 = z[0], a1 = z[1];
 #pragma unroll
@@ -477,21 +504,12 @@ __device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, 
     P.gamma = J.gamma[inst];
     const int j = role < 7 ? role + 1 : 7;
     const cx<double> *f = J.fpm + (int64_t)inst * 30;
-    P.f6[0] = f[0];
-    P.f6[1] = f[j];
-    P.f6[2] = f[8];
-    P.f6[3] = f[8 + j];
-    P.f6[4] = f[15 + j];
-    P.f6[5] = f[22 + j];
+    const int pidx[6] = {0, j, 8, 8 + j, 15 + j, 22 + j};  // P0, P_j, Pc0, Pc_j, Q_j, Qc_j
     if (J.spt) {
         const cx<double> *zs = J.spt + path * 21;
         const cx<double> *qs = J.spm + path * 30;
-        P.s6[0] = qs[0];
-        P.s6[1] = qs[j];
-        P.s6[2] = qs[8];
-        P.s6[3] = qs[8 + j];
-        P.s6[4] = qs[15 + j];
-        P.s6[5] = qs[22 + j];
+#pragma unroll
+        for (int p = 0; p < 6; ++p) P.par[p] = qs[pidx[p]];
         if (role < 7) {
             P.Yc[0] = zs[2 * role];
             P.Yc[1] = zs[2 * role + 1];
@@ -505,11 +523,16 @@ __device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, 
         generate_start(role, o, P);
     }
 #pragma unroll
-    for (int p = 0; p < 6; ++p) P.d6[p] = P.f6[p] - P.s6[p];
+    for (int p = 0; p < 6; ++p) P.par[6 + p] = f[pidx[p]] - P.par[p];
     // TrackPath reads the same stream from draw 0 (frg.cu:54,57)
     draw_vec(role, P, 0, o.patch_lo, o.patch_hi, P.u);
     switch_patch(role, P.u, P.Yc);
-    draw_vec(role, P, 46, o.patch_lo, o.patch_hi, P.Vc);
+    {
+        cx<double> v[3];
+        draw_vec(role, P, 46, o.patch_lo, o.patch_hi, v);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) P.vc[k] = v[k];
+    }
     P.draw = 92;
     P.phase = PH_INIT;
     P.stage = 0;
@@ -641,7 +664,7 @@ __device__ __forceinline__ void finish_newton(int role, const TrackOpts &o, Path
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             P.Yc[k] = P.K[0][k];
-            P.Vc[k] = P.K[7][k];
+            P.vc[k] = P.K[7][k];
         }
         P.streak++;
         if (P.streak < o.caution_steps && P.scale > 1.) P.scale = 1.;
@@ -750,7 +773,7 @@ __device__ __forceinline__ void finish_path(int role, const Job &J, PathState &P
     lds_sync();
     (void)cell;
     const cx<double> P0 = J.fpm[(int64_t)P.inst * 30 + 0], Pc0 = J.fpm[(int64_t)P.inst * 30 + 8];
-    const cx<double> Pj = P.f6[1];
+    const cx<double> Pj = J.fpm[(int64_t)P.inst * 30 + (role < 7 ? role + 1 : 7)];
     cx<double> c1[7], c2[7];
     cognate12(&lk[0], P0, c1);
     cognate12(&lk[7], Pc0, c2);
@@ -806,7 +829,7 @@ __device__ __forceinline__ void micro_step(int role, const TrackOpts &o, PathSta
         double c[7];
         const double s = P.s;
 #pragma unroll
-        for (int m = 0; m < 7; ++m) c[m] = rk ? rk_a(P.stage, m) * s : 0.0;
+        for (int m = 0; m < 7; ++m) c[m] = rk ? kA[P.stage & 7][m] * s : 0.0;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             cx<double> acc = P.Yc[k];
@@ -814,7 +837,7 @@ __device__ __forceinline__ void micro_step(int role, const TrackOpts &o, PathSta
             for (int m = 0; m < 7; ++m) acc = acc + c[m] * P.K[m][k];
             Ye[k] = sel(newton, P.K[0][k], acc);
             // w: Vc at init and stage 0, k_{stage-1} otherwise; e_t for Newton
-            cx<double> wv = P.Vc[k];
+            cx<double> wv = P.vc[k];
 #pragma unroll
             for (int m = 0; m < 7; ++m) wv = sel(rk && P.stage == m + 1, P.K[m][k], wv);
             const bool is_t = (role == 7 && k == 1);
@@ -846,20 +869,20 @@ __device__ __forceinline__ void micro_step(int role, const TrackOpts &o, PathSta
         e.U = role < 7 ? Ye[2] : mk(1.0, 0.0);
 #pragma unroll
         for (int p = 0; p < 6; ++p) {
-            e.s[p] = P.s6[p];
-            e.d[p] = P.d6[p];
+            e.s[p] = P.par[p];
+            e.d[p] = P.par[6 + p];
         }
         e.gamma = P.gamma;
         pose_values(e, fN[0], fN[1], fN[2]);
     }
     cx<double> x[3];
-    group_solve<R>(role, L, Ye, P.u, W, newton, w_rhs, P.s6, P.d6, P.gamma, fN, x);
+    group_solve<R>(role, L, Ye, P.u, W, newton, w_rhs, P.par, P.gamma, fN, x);
     if (!newton) {
         P.st_ivp++;
         normalize_tangent(role, x, P.tstep && P.phase != PH_INIT);
         if (P.phase == PH_INIT) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) P.Vc[k] = x[k];
+            for (int k = 0; k < 3; ++k) P.vc[k] = x[k];
             main_loop_top(o, P);
             return;
         }
@@ -913,8 +936,11 @@ __global__ __launch_bounds__(256) void track_kernel(Job J) {
     extern __shared__ unsigned char smem[];
     const int role = threadIdx.x & 7;
     const int grp = threadIdx.x >> 3;
-    GroupLds<double> *L = reinterpret_cast<GroupLds<double> *>(smem) + grp;
+    GroupSlot *S = reinterpret_cast<GroupSlot *>(smem) + grp;
+    GroupLds<double> *L = &S->lds;
     PathState P;
+    P.par = S->par[role];
+    P.vc = S->vc[role];
     int64_t path = kPersistent ? claim_path(role, J.queue) : (int64_t)blockIdx.x * (blockDim.x >> 3) + grp;
     if (path >= J.n_total) return;
     init_path(role, J, P, path);
@@ -942,7 +968,7 @@ __global__ __launch_bounds__(256) void track_kernel(Job J) {
 
 size_t track_lds_bytes(int block, int precision) {
     (void)precision;
-    return (size_t)(block / 8) * sizeof(GroupLds<double>);
+    return (size_t)(block / 8) * sizeof(GroupSlot);
 }
 
 int track_max_groups_per_cu(int block, int kind, int precision) {
@@ -1000,7 +1026,9 @@ __global__ __launch_bounds__(64) void debug_solve_kernel(DebugSolveArgs A) {
     extern __shared__ unsigned char smem[];
     const int role = threadIdx.x & 7;
     const int grp = threadIdx.x >> 3;
-    GroupLds<R> *L = reinterpret_cast<GroupLds<R> *>(smem) + grp;
+    GroupSlot *S = reinterpret_cast<GroupSlot *>(smem) + grp;
+    GroupLds<R> *L = reinterpret_cast<GroupLds<R> *>(&S->lds);
+    cx<double> *par = S->par[role];
     const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 3) + grp;
     if (p >= A.n) return;
     cx<double> Y[3], U[3], W[3], x[3], s6[6], d6[6];
@@ -1014,6 +1042,8 @@ __global__ __launch_bounds__(64) void debug_solve_kernel(DebugSolveArgs A) {
     for (int q = 0; q < 6; ++q) {
         s6[q] = sp[idx[q]];
         d6[q] = A.fpm[idx[q]] - s6[q];
+        par[q] = s6[q];
+        par[6 + q] = d6[q];
     }
     const bool newton = A.mode == 2;
     double w_rhs = 1.0;
@@ -1038,7 +1068,7 @@ __global__ __launch_bounds__(64) void debug_solve_kernel(DebugSolveArgs A) {
         e.gamma = A.gamma;
         pose_values(e, fN[0], fN[1], fN[2]);
     }
-    group_solve<R>(role, L, Y, U, W, newton, w_rhs, s6, d6, A.gamma, fN, x);
+    group_solve<R>(role, L, Y, U, W, newton, w_rhs, par, A.gamma, fN, x);
     if (!newton) normalize_tangent(role, x, A.mode == 1);
     store_slots(role, A.x + p * 23, x);
 }
@@ -1049,6 +1079,8 @@ __global__ __launch_bounds__(64) void debug_start_kernel(DebugStartArgs A) {
     const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 3) + grp;
     if (p >= A.n) return;
     PathState P;
+    cx<double> parbuf[12];
+    P.par = parbuf;
     P.seed = A.seed;
     P.pid = A.pid0 + (uint64_t)p;
     generate_start(role, A.o, P);
@@ -1059,14 +1091,14 @@ __global__ __launch_bounds__(64) void debug_start_kernel(DebugStartArgs A) {
         z[2 * role] = P.Yc[0];
         z[2 * role + 1] = P.Yc[1];
         z[14 + role] = P.Yc[2];
-        q[j] = P.s6[1];
-        q[8 + j] = P.s6[3];
-        q[15 + j] = P.s6[4];
-        q[22 + j] = P.s6[5];
+        q[j] = P.par[1];
+        q[8 + j] = P.par[3];
+        q[15 + j] = P.par[4];
+        q[22 + j] = P.par[5];
     }
     if (role == 0) {
-        q[0] = P.s6[0];
-        q[8] = P.s6[2];
+        q[0] = P.par[0];
+        q[8] = P.par[2];
     }
 }
 
@@ -1075,10 +1107,9 @@ hipError_t launch_debug_solve(const DebugSolveArgs &a, int precision, hipStream_
     const int grid = (int)((a.n + 7) / 8);
     if (grid == 0) return hipSuccess;
     if (precision) {
-        hipLaunchKernelGGL(debug_solve_kernel<float>, dim3(grid), dim3(block), 8 * sizeof(GroupLds<float>), stream, a);
+        hipLaunchKernelGGL(debug_solve_kernel<float>, dim3(grid), dim3(block), 8 * sizeof(GroupSlot), stream, a);
     } else {
-        hipLaunchKernelGGL(debug_solve_kernel<double>, dim3(grid), dim3(block), 8 * sizeof(GroupLds<double>), stream,
-                           a);
+        hipLaunchKernelGGL(debug_solve_kernel<double>, dim3(grid), dim3(block), 8 * sizeof(GroupSlot), stream, a);
     }
     return hipGetLastError();
 }
