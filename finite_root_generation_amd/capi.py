@@ -34,7 +34,9 @@ PATH_STATS_DTYPE = np.dtype([(f, np.int32) for f in PATH_STATS_FIELDS])
 EXPORTS = ["frg_default_opts", "frg_create", "frg_destroy", "frg_set_opts", "frg_track", "frg_track_device",
            "frg_track_explicit", "frg_track_batch", "frg_generate_starts", "frg_solve_step",
            "frg_sort_dedupe_merge", "frg_lambertw", "frg_completeness", "frg_strerror", "frg_version",
-           "frg_last_kernel_ms", "frg_flops"]
+           "frg_last_kernel_ms", "frg_flops", "frg_collect_step_log", "frg_train_step_model",
+           "frg_set_step_model", "frg_step_model_grad"]
+STEP_MODEL_SIZE, STEP_MODEL_WEIGHTS, STEP_RECORD = 4819, 4801, 9
 
 
 class C64(ct.Structure):
@@ -127,6 +129,11 @@ def lib() -> ct.CDLL:
         L.frg_version.restype = ct.c_int
         L.frg_last_kernel_ms.argtypes = [P, ct.POINTER(ct.c_double)]
         L.frg_flops.argtypes = [ct.POINTER(FlopModel)]
+        L.frg_collect_step_log.argtypes = [P, P, C64, ct.c_uint64, ct.c_uint64, ct.c_int64, P, ct.c_int64,
+                                           ct.POINTER(ct.c_int64)]
+        L.frg_train_step_model.argtypes = [P, P, ct.c_int64, ct.c_int32, ct.c_int32, ct.c_float, ct.c_uint64, P, P]
+        L.frg_set_step_model.argtypes = [P, P]
+        L.frg_step_model_grad.argtypes = [P, P, P, ct.c_int64, P, P]
         _lib = L
     return _lib
 
@@ -248,6 +255,36 @@ class Tracker:
         v = ct.c_double()
         _check(lib().frg_last_kernel_ms(self._h, ct.byref(v)), "frg_last_kernel_ms")
         return v.value
+
+    def collect_step_log(self, fpm, gamma, seed: int, n: int, cap: int, first_path_id: int = 0):
+        fpm = _c(fpm, (NP,))
+        rec = np.zeros((cap, STEP_RECORD), np.float32)
+        got = ct.c_int64()
+        _check(lib().frg_collect_step_log(self._h, _p(fpm), _c64(gamma), seed, first_path_id, n, _p(rec), cap,
+                                          ct.byref(got)), "frg_collect_step_log")
+        return rec[:got.value]
+
+    def train_step_model(self, records, steps: int = 2000, batch: int = 4096, lr: float = 2e-3, seed: int = 0):
+        rec = np.ascontiguousarray(records, np.float32)
+        model = np.zeros(STEP_MODEL_SIZE, np.float32)
+        loss = np.zeros(max(steps, 1), np.float32)
+        _check(lib().frg_train_step_model(self._h, _p(rec), rec.shape[0], steps, batch, lr, seed, _p(model),
+                                          _p(loss)), "frg_train_step_model")
+        return model, loss[:steps]
+
+    def set_step_model(self, model):
+        m = np.ascontiguousarray(model, np.float32)
+        assert m.size == STEP_MODEL_SIZE
+        _check(lib().frg_set_step_model(self._h, _p(m)), "frg_set_step_model")
+
+    def step_model_grad(self, model, records):
+        m = np.ascontiguousarray(model, np.float32)
+        rec = np.ascontiguousarray(records, np.float32)
+        g = np.zeros(STEP_MODEL_WEIGHTS, np.float32)
+        loss = ct.c_float()
+        _check(lib().frg_step_model_grad(self._h, _p(m), _p(rec), rec.shape[0], _p(g), ct.byref(loss)),
+               "frg_step_model_grad")
+        return g, loss.value
 
     def generate_starts(self, seed: int, n: int, first_path_id: int = 0):
         spt = np.zeros((n, NV), np.complex128)

@@ -7,12 +7,14 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <vector>
 
 #include "../../include/frg.h"
 #include "frg_kernels.h"
+#include "frg_mlp_kernels.h"
 
 using frg::cx;
 
@@ -49,6 +51,12 @@ struct frg_ctx {
     frg_opts opts{};
     DevBuf inst_first, fpm, gamma, seed, pid0, spt, spm, fpt, err, stats, yfinal, queue, dbg_a, dbg_b, dbg_c,
         dbg_d, dbg_e, dbg_x;
+    // step model: FP16 weights on the device, normalisation on the host
+    DevBuf model16, steplog, steplog_count;
+    bool have_model = false;
+    frg::StepModel model{};
+    // training state
+    DevBuf tw, tm, tv, tgrad, tsamples, tnorm, tloss, tidx, tw16;
 };
 
 namespace {
@@ -91,7 +99,7 @@ int check_opts(const frg_opts &o) {
     if (o.start_mode != FRG_START_UNIFORM && o.start_mode != FRG_START_GAUSSIAN) return FRG_EINVAL;
     if (o.precision != FRG_PREC_FP64 && o.precision != FRG_PREC_MIXED) return FRG_EINVAL;
     if (o.schedule != FRG_SCHED_STATIC && o.schedule != FRG_SCHED_PERSISTENT) return FRG_EINVAL;
-    if (o.step_control != FRG_STEP_CLASSIC) return FRG_EUNSUPPORTED;
+    if (o.step_control != FRG_STEP_CLASSIC && o.step_control != FRG_STEP_LEARNED) return FRG_EINVAL;
     if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 256 || o.block_size % 64 != 0)) return FRG_EINVAL;
     if (o.groups_per_cu < 0 || o.max_steps < 0 || o.max_ministeps < 0) return FRG_EINVAL;
     if (!(o.smin > 0.0) || !(o.rk_tol > 0.0)) return FRG_EINVAL;
@@ -104,6 +112,9 @@ int pick_block(const frg_ctx *c) { return c->opts.block_size ? c->opts.block_siz
 
 int run_job(frg_ctx *c, frg::Job &job, hipStream_t stream, frg_stats *summary) {
     job.o = to_track_opts(c->opts);
+    job.learned = c->opts.step_control == FRG_STEP_LEARNED;
+    if (job.learned && !c->have_model) return FRG_EINVAL;
+    job.model = c->model;
     const int block = pick_block(c);
     const int gpb = block / 8;
     int grid;
@@ -320,7 +331,8 @@ void frg_destroy(frg_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     DevBuf *bufs[] = {&c->inst_first, &c->fpm, &c->gamma, &c->seed, &c->pid0, &c->spt, &c->spm, &c->fpt, &c->err,
                       &c->stats, &c->yfinal, &c->queue, &c->dbg_a, &c->dbg_b, &c->dbg_c, &c->dbg_d, &c->dbg_e,
-                      &c->dbg_x};
+                      &c->dbg_x, &c->model16, &c->steplog, &c->steplog_count, &c->tw, &c->tm, &c->tv, &c->tgrad,
+                      &c->tsamples, &c->tnorm, &c->tloss, &c->tidx, &c->tw16};
     for (DevBuf *b : bufs) b->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -463,6 +475,203 @@ int frg_solve_step(frg_ctx *c, int32_t mode, int32_t precision, int64_t n, const
     HIPCHK(frg::launch_debug_solve(a, precision, c->stream));
     HIPCHK(hipMemcpyAsync(x, c->dbg_x.p, v23, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return FRG_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// Learned step control
+// ------------------------------------------------------------------------------------------
+namespace {
+
+void fill_model_norm(frg::StepModel &m, const float *model) {
+    for (int i = 0; i < frg::kMlpIn; ++i) {
+        m.xmu[i] = model[frg::kMlpXmu + i];
+        m.xsd[i] = model[frg::kMlpXsd + i];
+    }
+    m.ymu = model[frg::kMlpYmu];
+    m.ysd = model[frg::kMlpYsd];
+}
+
+// mean and deviation of every feature and of the target (deviation floored at 1e-3)
+void record_stats(const float *rec, int64_t n, float *norm) {
+    for (int q = 0; q <= frg::kMlpIn; ++q) {
+        double s = 0, s2 = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            const double v = rec[i * frg::kStepRecord + q];
+            s += v;
+            s2 += v * v;
+        }
+        const double mu = s / (double)n;
+        const double sd = std::sqrt(std::max(s2 / (double)n - mu * mu, 1e-6));
+        const int mi = q < frg::kMlpIn ? q : 2 * frg::kMlpIn;
+        const int si = q < frg::kMlpIn ? frg::kMlpIn + q : 2 * frg::kMlpIn + 1;
+        norm[mi] = (float)mu;
+        norm[si] = (float)std::max(sd, 1e-3);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int frg_collect_step_log(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
+                         int64_t n_paths, float *records, int64_t cap, int64_t *n_records) {
+    if (!c || !fpm || n_paths < 0 || cap < 0 || (cap > 0 && !records) || !n_records) return FRG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    frg::Job job{};
+    int rc = upload_instances(c, 1, fpm, &gamma, &seed, &first_path_id, &n_paths, job, c->stream);
+    if (rc != FRG_OK) return rc;
+    HIPCHK(c->fpt.ensure(sizeof(frg_c64) * 21 * std::max<int64_t>(n_paths, 1)));
+    HIPCHK(c->err.ensure(sizeof(int32_t) * std::max<int64_t>(n_paths, 1)));
+    HIPCHK(c->steplog.ensure(sizeof(float) * frg::kStepRecord * std::max<int64_t>(cap, 1)));
+    HIPCHK(c->steplog_count.ensure(sizeof(unsigned long long)));
+    HIPCHK(frg::launch_reset_counter(c->steplog_count.as<unsigned long long>(), c->stream));
+    job.fpt = c->fpt.as<cx<double>>();
+    job.err = c->err.as<int>();
+    job.steplog = c->steplog.as<float>();
+    job.steplog_count = c->steplog_count.as<unsigned long long>();
+    job.steplog_cap = cap;
+    rc = run_job(c, job, c->stream, nullptr);
+    if (rc != FRG_OK) return rc;
+    unsigned long long cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, c->steplog_count.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int64_t got = std::min<int64_t>((int64_t)cnt, cap);
+    if (got > 0)
+        HIPCHK(hipMemcpy(records, c->steplog.p, sizeof(float) * frg::kStepRecord * got, hipMemcpyDeviceToHost));
+    *n_records = got;
+    return FRG_OK;
+}
+
+int frg_set_step_model(frg_ctx *c, const float *model) {
+    if (!c || !model) return FRG_EINVAL;
+    for (int i = 0; i < FRG_STEP_MODEL_SIZE; ++i)
+        if (!std::isfinite(model[i])) return FRG_EINVAL;
+    for (int i = 0; i < frg::kMlpIn; ++i)
+        if (!(model[frg::kMlpXsd + i] > 0.f)) return FRG_EINVAL;
+    if (!(model[frg::kMlpYsd] > 0.f)) return FRG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(c->tw.ensure(sizeof(float) * frg::kMlpNW));
+    HIPCHK(c->model16.ensure(sizeof(__half) * frg::kMlpNW));
+    HIPCHK(hipMemcpyAsync(c->tw.p, model, sizeof(float) * frg::kMlpNW, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(frg::launch_mlp_to_half(c->tw.as<float>(), c->model16.as<__half>(), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    fill_model_norm(c->model, model);
+    c->model.w = c->model16.as<__half>();
+    c->have_model = true;
+    return FRG_OK;
+}
+
+int frg_step_model_grad(frg_ctx *c, const float *model, const float *records, int64_t n, float *grad, float *loss) {
+    if (!c || !model || !records || n <= 0 || !grad) return FRG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(c->tw.ensure(sizeof(float) * frg::kMlpNW));
+    HIPCHK(c->tgrad.ensure(sizeof(float) * frg::kMlpNW));
+    HIPCHK(c->tsamples.ensure(sizeof(float) * frg::kStepRecord * n));
+    HIPCHK(c->tnorm.ensure(sizeof(float) * 18));
+    HIPCHK(c->tloss.ensure(sizeof(float)));
+    HIPCHK(c->tidx.ensure(sizeof(int64_t) * n));
+    std::vector<int64_t> idx((size_t)n);
+    for (int64_t i = 0; i < n; ++i) idx[i] = i;
+    HIPCHK(hipMemcpyAsync(c->tw.p, model, sizeof(float) * frg::kMlpNW, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->tnorm.p, model + frg::kMlpXmu, sizeof(float) * 18, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->tsamples.p, records, sizeof(float) * frg::kStepRecord * n, hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipMemcpyAsync(c->tidx.p, idx.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->tgrad.p, 0, sizeof(float) * frg::kMlpNW, c->stream));
+    HIPCHK(hipMemsetAsync(c->tloss.p, 0, sizeof(float), c->stream));
+    frg::MlpGradArgs a{};
+    a.w = c->tw.as<float>();
+    a.samples = c->tsamples.as<float>();
+    a.norm = c->tnorm.as<float>();
+    a.indices = c->tidx.as<int64_t>();
+    a.n_samples = n;
+    a.batch = n;
+    a.grad = c->tgrad.as<float>();
+    a.loss = c->tloss.as<float>();
+    HIPCHK(frg::launch_mlp_grad(a, c->stream));
+    HIPCHK(hipMemcpyAsync(grad, c->tgrad.p, sizeof(float) * frg::kMlpNW, hipMemcpyDeviceToHost, c->stream));
+    float l = 0.f;
+    HIPCHK(hipMemcpyAsync(&l, c->tloss.p, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (loss) *loss = l;
+    return FRG_OK;
+}
+
+int frg_train_step_model(frg_ctx *c, const float *records, int64_t n, int32_t steps, int32_t batch, float lr,
+                         uint64_t seed, float *model, float *loss) {
+    if (!c || !records || n <= 0 || steps < 0 || batch <= 0 || !(lr > 0.f) || !model) return FRG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    // normalisation from the data; Glorot-uniform initial weights from a host generator
+    std::vector<float> h((size_t)FRG_STEP_MODEL_SIZE, 0.f);
+    record_stats(records, n, h.data() + frg::kMlpXmu);
+    uint64_t st = seed * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull;
+    auto urand = [&]() {  // xorshift64*
+        st ^= st >> 12;
+        st ^= st << 25;
+        st ^= st >> 27;
+        return (double)((st * 0x2545F4914F6CDD1Dull) >> 11) * 0x1.0p-53;
+    };
+    const float a1 = std::sqrt(6.f / (frg::kMlpIn + frg::kMlpH)), a2 = std::sqrt(6.f / (2 * frg::kMlpH)),
+                a3 = std::sqrt(6.f / (frg::kMlpH + 1));
+    for (int i = 0; i < frg::kMlpH * frg::kMlpIn; ++i) h[frg::kMlpW1 + i] = (float)((2 * urand() - 1) * a1);
+    for (int i = 0; i < frg::kMlpH * frg::kMlpH; ++i) h[frg::kMlpW2 + i] = (float)((2 * urand() - 1) * a2);
+    for (int i = 0; i < frg::kMlpH; ++i) h[frg::kMlpW3 + i] = (float)((2 * urand() - 1) * a3);
+    const size_t wbytes = sizeof(float) * frg::kMlpNW;
+    HIPCHK(c->tw.ensure(wbytes));
+    HIPCHK(c->tm.ensure(wbytes));
+    HIPCHK(c->tv.ensure(wbytes));
+    HIPCHK(c->tgrad.ensure(wbytes));
+    HIPCHK(c->tw16.ensure(sizeof(__half) * frg::kMlpNW));
+    HIPCHK(c->tsamples.ensure(sizeof(float) * frg::kStepRecord * n));
+    HIPCHK(c->tnorm.ensure(sizeof(float) * 18));
+    HIPCHK(c->tloss.ensure(sizeof(float) * std::max(steps, 1)));
+    HIPCHK(hipMemcpyAsync(c->tw.p, h.data(), wbytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->tm.p, 0, wbytes, c->stream));
+    HIPCHK(hipMemsetAsync(c->tv.p, 0, wbytes, c->stream));
+    HIPCHK(hipMemsetAsync(c->tgrad.p, 0, wbytes, c->stream));
+    HIPCHK(hipMemsetAsync(c->tloss.p, 0, sizeof(float) * std::max(steps, 1), c->stream));
+    HIPCHK(hipMemcpyAsync(c->tnorm.p, h.data() + frg::kMlpXmu, sizeof(float) * 18, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->tsamples.p, records, sizeof(float) * frg::kStepRecord * n, hipMemcpyHostToDevice,
+                          c->stream));
+    const float b1 = 0.9f, b2 = 0.999f;
+    float b1t = 1.f, b2t = 1.f;
+    for (int k = 0; k < steps; ++k) {
+        frg::MlpGradArgs g{};
+        g.w = c->tw.as<float>();
+        g.samples = c->tsamples.as<float>();
+        g.norm = c->tnorm.as<float>();
+        g.indices = nullptr;
+        g.n_samples = n;
+        g.batch = batch;
+        g.seed = seed;
+        g.step = (uint64_t)k;
+        g.grad = c->tgrad.as<float>();
+        g.loss = c->tloss.as<float>() + k;
+        HIPCHK(frg::launch_mlp_grad(g, c->stream));
+        b1t *= b1;
+        b2t *= b2;
+        frg::MlpAdamArgs ad{};
+        ad.w = c->tw.as<float>();
+        ad.m = c->tm.as<float>();
+        ad.v = c->tv.as<float>();
+        ad.grad = c->tgrad.as<float>();
+        ad.w16 = c->tw16.as<__half>();
+        ad.lr = lr;
+        ad.b1 = b1;
+        ad.b2 = b2;
+        ad.eps = 1e-8f;
+        ad.b1t = b1t;
+        ad.b2t = b2t;
+        HIPCHK(frg::launch_mlp_adam(ad, c->stream));
+    }
+    HIPCHK(hipMemcpyAsync(h.data(), c->tw.p, wbytes, hipMemcpyDeviceToHost, c->stream));
+    if (loss && steps > 0)
+        HIPCHK(hipMemcpyAsync(loss, c->tloss.p, sizeof(float) * steps, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::memcpy(model, h.data(), sizeof(float) * FRG_STEP_MODEL_SIZE);
     return FRG_OK;
 }
 

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "frg_device.h"
+#include "frg_mlp.h"
 
 namespace frg {
 
@@ -41,6 +42,13 @@ struct Job {
     cx<double> *yfinal;           // n_total x 23 homogeneous end points or null
     unsigned long long *queue;    // work-queue head (persistent launches)
     TrackOpts o;
+    // learned step control
+    int learned;                  // 1: the model proposes the step after an accepted step
+    StepModel model;
+    // step log: (features of a proposal, log of the step that met the tolerance next)
+    float *steplog;               // steplog_cap x kStepRecord, or null
+    unsigned long long *steplog_count;
+    int64_t steplog_cap;
 };
 
 // Stage-level parity entry points (one group solve per path; start generation alone).

@@ -351,6 +351,10 @@ struct PathState {
     int64_t path;
     int inst;
     int st_rk, st_newton_calls, st_newton_its, st_redraws, st_ivp, st_fp32;
+    // learned step control: features of the last proposal, rejections in the current step
+    float fprev[kMlpIn];
+    bool have_prev;
+    int rej;
 };
 
 // Patch draw (htpytrack.cu:55-58,147-150): Y index i uses draws base + 2i, base + 2i + 1.
@@ -552,6 +556,8 @@ __device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, 
     P.st_rk = P.st_newton_calls = P.st_newton_its = P.st_redraws = 0;
     P.st_ivp = 0;
     P.st_fp32 = 0;
+    P.have_prev = false;
+    P.rej = 0;
 #pragma unroll
     for (int m = 0; m < 8; ++m)
 #pragma unroll
@@ -640,6 +646,7 @@ __device__ __forceinline__ void finish_rk(int role, const TrackOpts &o, PathStat
         P.s *= scale;
         if (P.s < o.smin) P.s = o.smin * 0.9;
         P.streak = 0;
+        P.rej++;
         start_rk(P);
         return;
     }
@@ -653,28 +660,90 @@ __device__ __forceinline__ void finish_rk(int role, const TrackOpts &o, PathStat
     start_rk(P);
 }
 // end of a corrector Newton call (:140-164)
-__device__ __forceinline__ void finish_newton(int role, const TrackOpts &o, PathState &P) {
+// Features of the step proposal made at an accepted step (see frg_mlp.h), identical in every
+// lane of the group.
+__device__ __forceinline__ void step_features(int role, const PathState &P, float f[kMlpIn]) {
+    const double t = grp_t(role, P.Yc);
+    const double ynorm = grp_norm(P.Yc);
+    cx<double> v[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] = P.vc[k];
+    const double vt = fabs(grp_bcast(v[1].re, role, 7));
+    f[0] = (float)t;
+    f[1] = (float)log(P.s);
+    f[2] = (float)log(fmax(P.err, 1e-30));
+    f[3] = (float)P.its;
+    f[4] = (float)log(fmax(P.residual, 1e-30));
+    f[5] = (float)log(fmax(ynorm, 1e-300));
+    f[6] = (float)(-log(fmax(vt, 1e-300)));
+    f[7] = (float)P.rej;
+}
+
+// Accepted step (htpytrack.cu:176-183).  The next step size comes from the classical rule
+// s *= scale (scale = 0.8 (tol/err)^(1/6), capped at 5, at most 1 within cautionSteps of a
+// rejection), or -- learned control, arclength steps -- from the model, with the same cap and
+// caution rule and the classical value as the fallback for a non-finite proposal.
+__device__ __forceinline__ void accept_step(int role, const Job &J, PathState &P, float *h1buf) {
+    const TrackOpts &o = J.o;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        P.Yc[k] = P.K[0][k];
+        P.vc[k] = P.K[7][k];
+    }
+    const bool use_model = (J.learned || J.steplog) && !P.tstep;
+    float f[kMlpIn];
+    if (use_model) step_features(role, P, f);
+    if (J.steplog && P.have_prev && !P.tstep && P.err > 0.0) {
+        // outcome of the previous proposal: the step that would have met the tolerance now
+        const float target = (float)(log(P.s) + log(o.rk_tol / P.err) / 6.0);
+        if (role == 0) {
+            const unsigned long long i = atomicAdd(J.steplog_count, 1ull);
+            if ((int64_t)i < J.steplog_cap) {
+                float *rec = J.steplog + (int64_t)i * kStepRecord;
+#pragma unroll
+                for (int q = 0; q < kMlpIn; ++q) rec[q] = P.fprev[q];
+                rec[kMlpIn] = target;
+            }
+        }
+    }
+    P.streak++;
+    if (P.streak < o.caution_steps && P.scale > 1.) P.scale = 1.;
+    double s_next = P.s * P.scale;
+    if (J.learned && use_model) {
+        const float y = mlp_eval_group(role, J.model, f, h1buf);
+        double s_ml = exp((double)y) * 0.85;
+        if (s_ml == s_ml && s_ml < 1e300) {
+            s_ml = fmin(s_ml, 5.0 * P.s);
+            if (P.streak < o.caution_steps) s_ml = fmin(s_ml, P.s);
+            s_next = s_ml;
+        }
+    }
+    P.s = s_next;
+    if (P.s > o.smax) P.s = o.smax;
+    else if (P.s < o.smin) P.s = o.smin * 0.9;
+    if (use_model) {
+#pragma unroll
+        for (int q = 0; q < kMlpIn; ++q) P.fprev[q] = f[q];
+    }
+    P.have_prev = use_model;
+    P.rej = 0;
+    P.step++;
+    main_loop_top(o, P);
+}
+
+__device__ __forceinline__ void finish_newton(int role, const Job &J, PathState &P, float *h1buf) {
+    const TrackOpts &o = J.o;
     P.st_newton_calls++;
     if (P.ministep++ == o.max_ministeps) {
         P.code = 2;
         P.done = true;
         return;
     }
-    if (P.residual < o.newt_tol) {  // accept (:176-183)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            P.Yc[k] = P.K[0][k];
-            P.vc[k] = P.K[7][k];
-        }
-        P.streak++;
-        if (P.streak < o.caution_steps && P.scale > 1.) P.scale = 1.;
-        P.s *= P.scale;
-        if (P.s > o.smax) P.s = o.smax;
-        else if (P.s < o.smin) P.s = o.smin * 0.9;
-        P.step++;
-        main_loop_top(o, P);
+    if (P.residual < o.newt_tol) {
+        accept_step(role, J, P, h1buf);
         return;
     }
+    P.rej++;
     redraw_patch(role, o, P);
     P.attempt++;
     if (P.attempt < o.newt_attempts) {
@@ -819,7 +888,8 @@ __device__ __forceinline__ void finish_path(int role, const Job &J, PathState &P
 // One solve for the current phase, then the transition.
 // ------------------------------------------------------------------------------------------
 template <typename R>
-__device__ __forceinline__ void micro_step(int role, const TrackOpts &o, PathState &P, GroupLds<R> *L) {
+__device__ __forceinline__ void micro_step(int role, const Job &J, PathState &P, GroupLds<R> *L) {
+    const TrackOpts &o = J.o;
     // evaluation point, w row and right-hand side of the current phase
     const bool newton = (P.phase == PH_NEWTON || P.phase == PH_SHARP);
     const bool rk = (P.phase == PH_RK);
@@ -905,7 +975,7 @@ __device__ __forceinline__ void micro_step(int role, const TrackOpts &o, PathSta
     const double tol = P.phase == PH_SHARP ? o.sharp_tol : o.newt_tol;
     if (P.its < max_its && P.residual > tol) return;  // another iteration
     if (P.phase == PH_SHARP) finish_sharp(role, o, P);
-    else finish_newton(role, o, P);
+    else finish_newton(role, J, P, reinterpret_cast<float *>(L->a));
 }
 
 // Next path for a persistent group: lane 0 takes a ticket from the device queue.
@@ -917,11 +987,12 @@ __device__ __forceinline__ int64_t claim_path(int role, unsigned long long *queu
     return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
-// Wave-uniform precision choice for mixed mode: FP32 solve unless some group of the wave needs
-// FP64 (initial tangent, sharpening, or within fp64_band of t = 1).
+// Wave-uniform precision choice for mixed mode: the RK stage tangents are solved in FP32 unless
+// some group of the wave needs FP64 (initial tangent, any Newton step -- corrector and
+// sharpening -- or a stage within fp64_band of t = 1).
 __device__ __forceinline__ bool needs_fp64(int role, const TrackOpts &o, const PathState &P) {
     if (o.precision == 0) return true;
-    if (P.phase == PH_INIT || P.phase == PH_SHARP) return true;
+    if (P.phase != PH_RK) return true;
     const double tc = grp_t(role, P.Yc);
     return (1.0 - tc) < o.fp64_band;
 }
@@ -948,13 +1019,13 @@ __global__ __launch_bounds__(256) void track_kernel(Job J) {
         if (kMixed) {
             const bool need64 = needs_fp64(role, J.o, P);
             if (__any(need64)) {
-                micro_step<double>(role, J.o, P, L);
+                micro_step<double>(role, J, P, L);
             } else {
                 P.st_fp32++;
-                micro_step<float>(role, J.o, P, reinterpret_cast<GroupLds<float> *>(L));
+                micro_step<float>(role, J, P, reinterpret_cast<GroupLds<float> *>(L));
             }
         } else {
-            micro_step<double>(role, J.o, P, L);
+            micro_step<double>(role, J, P, L);
         }
         if (P.done) {
             finish_path(role, J, P, L);

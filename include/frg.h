@@ -149,6 +149,32 @@ int64_t frg_sort_dedupe_merge(const frg_c64 *coll, int64_t m, const frg_c64 *fpt
 double frg_lambertw(double z);
 double frg_completeness(int64_t successes, int64_t duplicates);
 
+/* ---- Learned step-size control (no reference counterpart: the reference's controller is the
+ * classical rule of rungekutta.cu:109-112 and htpytrack.cu:179-182, which stays the fallback).
+ * A model is an 8 -> 64 -> 64 -> 1 tanh MLP (4801 weights) plus input/output normalisation:
+ *   W1[64][8], b1[64], W2[64][64], b2[64], w3[64], b3, xmu[8], xsd[8], ymu, ysd  (FP32)
+ * A step record is 8 features of a step proposal and the log of the step that met the RK
+ * tolerance on the next accepted step. */
+#define FRG_STEP_MODEL_SIZE 4819
+#define FRG_STEP_MODEL_WEIGHTS 4801
+#define FRG_STEP_RECORD 9
+
+/* Tracks n_paths paths (as frg_track, outputs discarded) and returns up to cap step records. */
+int frg_collect_step_log(frg_ctx *ctx, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
+                         int64_t n_paths, float *records, int64_t cap, int64_t *n_records);
+
+/* Trains a model on n records with the GPU kernels (minibatch Adam, `steps` steps of `batch`
+ * records drawn with replacement from seed).  model: FRG_STEP_MODEL_SIZE floats out; loss
+ * (may be NULL): the minibatch loss of every step. */
+int frg_train_step_model(frg_ctx *ctx, const float *records, int64_t n, int32_t steps, int32_t batch, float lr,
+                         uint64_t seed, float *model, float *loss);
+
+/* Loads a model for FRG_STEP_LEARNED tracking (weights stored as FP16 on the device). */
+int frg_set_step_model(frg_ctx *ctx, const float *model);
+
+/* Gradient of the mean loss over records 0..n-1 with respect to the 4801 weights (GPU). */
+int frg_step_model_grad(frg_ctx *ctx, const float *model, const float *records, int64_t n, float *grad, float *loss);
+
 /* Device time (ms, HIP events) of the last tracking launch of the context; waits for it to end.
  * No reference counterpart (the reference times RNG init + kernel + copies, frg.cu:356-426). */
 int frg_last_kernel_ms(frg_ctx *ctx, double *ms);

@@ -1,0 +1,94 @@
+"""GPU tests of learned step control: the training kernels' gradient against the numpy oracle,
+training progress, the classical fallback, and what the trained model gains on a held-out task
+with the RK error test still accepting or rejecting every step."""
+import sys
+
+import numpy as np
+import pytest
+
+from finite_root_generation_amd import capi, tasks
+from oracle import mlp_oracle as M
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def records(tracker):
+    fpm, g = tasks.seeded_task(10)
+    say("[gpu] collect step log n=2048")
+    tracker.set_opts(step_control=capi.STEP_CLASSIC, schedule=capi.SCHED_STATIC)
+    rec = tracker.collect_step_log(fpm, g, seed=3, n=2048, cap=400_000)
+    assert rec.shape[0] > 100_000 and np.all(np.isfinite(rec))
+    return rec
+
+
+def test_grad_kernel_matches_oracle(tracker, records):
+    rng = np.random.default_rng(0)
+    m = M.random_model(rng, records).astype(np.float32)
+    sub = records[:3000]
+    say("[gpu] step model gradient")
+    g, loss = tracker.step_model_grad(m, sub)
+    lo, go = M.loss_and_grad(m.astype(np.float64), sub)
+    assert abs(loss - lo) <= 1e-4 * abs(lo)
+    assert np.linalg.norm(g - go) <= 1e-4 * np.linalg.norm(go)
+
+
+def test_training_reduces_loss(tracker, records):
+    say("[gpu] train 800 steps")
+    model, loss = tracker.train_step_model(records, steps=800, batch=2048, lr=2e-3, seed=1)
+    assert np.all(np.isfinite(model)) and np.all(np.isfinite(loss))
+    assert loss[-50:].mean() < 0.25 * loss[:10].mean()
+    # the trained model beats predicting the mean target (normalised loss 0.5) on held-out data
+    lo, _ = M.loss_and_grad(model.astype(np.float64), records[-20000:])
+    assert lo < 0.1
+
+
+def test_fallback_to_classical(tracker, records):
+    """A model whose proposal overflows falls back to the classical rule: identical paths."""
+    fpm, g = tasks.seeded_task(12)
+    n = 512
+    bad = M.random_model(np.random.default_rng(2), records).astype(np.float32)
+    bad[M.YMU] = 1e6
+    tracker.set_opts(step_control=capi.STEP_CLASSIC)
+    a = tracker.track(fpm, g, seed=5, n=n)
+    tracker.set_step_model(bad)
+    tracker.set_opts(step_control=capi.STEP_LEARNED)
+    try:
+        say("[gpu] learned with overflowing model")
+        b = tracker.track(fpm, g, seed=5, n=n)
+    finally:
+        tracker.set_opts(step_control=capi.STEP_CLASSIC)
+    assert np.array_equal(a[1], b[1])
+    assert np.array_equal(a[0].view(np.float64), b[0].view(np.float64), equal_nan=True)
+
+
+def test_learned_control_gain_on_held_out_task(tracker, records):
+    model, _ = tracker.train_step_model(records, steps=2000, batch=4096, lr=2e-3, seed=1)
+    fpm, g = tasks.seeded_task(11)
+    n = 8192
+    res = {}
+    for name, sc in (("classical", capi.STEP_CLASSIC), ("learned", capi.STEP_LEARNED)):
+        if sc == capi.STEP_LEARNED:
+            tracker.set_step_model(model)
+        tracker.set_opts(step_control=sc, schedule=capi.SCHED_PERSISTENT)
+        say(f"[gpu] {name} n={n}")
+        res[name] = tracker.track(fpm, g, seed=4, n=n)
+    tracker.set_opts(step_control=capi.STEP_CLASSIC, schedule=capi.SCHED_STATIC)
+    (fa, ea, sa, _), (fb, eb, sb, _) = res["classical"], res["learned"]
+    ma, mb = tasks.finite_mask(fa, ea), tasks.finite_mask(fb, eb)
+    both = ma & mb
+    d = np.max(np.abs(fb[both] - fa[both]) / np.maximum(1.0, np.abs(fa[both])), axis=1)
+    recall = (d <= 1e-8).sum() / ma.sum()
+    rk_a, rk_b = sa["rk_attempts"].mean(), sb["rk_attempts"].mean()
+    solves_a = (sa["tangent_solves"] + sa["newton_its"]).mean()
+    solves_b = (sb["tangent_solves"] + sb["newton_its"]).mean()
+    say(f"    classical rk/path {rk_a:.2f} solves/path {solves_a:.1f}; learned rk/path {rk_b:.2f} "
+        f"solves/path {solves_b:.1f}; recall {recall:.4f}; finite {ma.sum()} vs {mb.sum()}")
+    assert recall >= 0.99
+    assert abs(int(mb.sum()) - int(ma.sum())) <= 0.005 * n
+    assert rk_b < rk_a  # fewer RK attempts per path
