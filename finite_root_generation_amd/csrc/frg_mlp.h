@@ -38,35 +38,84 @@ struct StepModel {
     float xmu[kMlpIn], xsd[kMlpIn], ymu, ysd;
 };
 
-// Group-cooperative evaluation (8 lanes, each owning 8 hidden units per layer).  `x` holds the
-// raw features, identical in all lanes; `h1buf` is 64 floats of this group's LDS.  Returns the
-// predicted log step size in every lane.
-__device__ __forceinline__ float mlp_eval_group(int role, const StepModel &M, const float x[kMlpIn], float *h1buf) {
-    float xn[kMlpIn];
+__device__ __forceinline__ float wave_sum(float v) {
+    v = grp_sum(v);
+    v += __shfl_xor(v, 8);
+    v += __shfl_xor(v, 16);
+    v += __shfl_xor(v, 32);
+    return v;
+}
+
+__device__ __forceinline__ void unpack8(uint4 q, float out[8]) {
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-    for (int i = 0; i < kMlpIn; ++i) xn[i] = (x[i] - M.xmu[i]) / M.xsd[i];
-    const __half *W1 = M.w + kMlpW1, *B1 = M.w + kMlpB1, *W2 = M.w + kMlpW2, *B2 = M.w + kMlpB2, *W3 = M.w + kMlpW3;
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-        const int h = 8 * role + u;
-        float a = __half2float(B1[h]);
-#pragma unroll
-        for (int i = 0; i < kMlpIn; ++i) a = fmaf(__half2float(W1[h * kMlpIn + i]), xn[i], a);
-        h1buf[h] = fast_tanh(a);
+    for (int i = 0; i < 4; ++i) {
+        const __half2 h = *reinterpret_cast<const __half2 *>(&w[i]);
+        out[2 * i] = __low2float(h);
+        out[2 * i + 1] = __high2float(h);
     }
-    lds_sync();
-    float part = 0.f;
+}
+
+// Wave-wide evaluation for the groups of one wave that have a pending proposal.  Lane l owns
+// hidden unit l of both layers, so its weights are one row of W1 and one row of W2 (read once
+// per call); the groups are then evaluated one after the other.  Must be called by all 64 lanes
+// with a wave-uniform `lanes` mask whose set bits are lanes 8g of the pending groups g.  Group g
+// has written its normalised features to area0[g * stride + 0..7] (floats in LDS); the same area
+// holds that group's first hidden layer at offset 64.  `gw` is this lane's group in the wave.
+// Returns the predicted log step size in the lanes of each pending group.
+__device__ __forceinline__ float mlp_eval_wave(const StepModel &M, float *area0, int stride, uint64_t lanes, int gw) {
+    const int l = __lane_id();
+    const uint4 *w1 = reinterpret_cast<const uint4 *>(M.w + kMlpW1 + kMlpIn * l);
+    const uint4 *w2 = reinterpret_cast<const uint4 *>(M.w + kMlpW2 + kMlpH * l);
+    float a1[8];
+    unpack8(w1[0], a1);
+    uint4 r2[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-        const int h = 8 * role + u;
-        float a = __half2float(B2[h]);
-#pragma unroll 16
-        for (int k = 0; k < kMlpH; ++k) a = fmaf(__half2float(W2[h * kMlpH + k]), h1buf[k], a);
-        part = fmaf(__half2float(W3[h]), fast_tanh(a), part);
-    }
+    for (int i = 0; i < 8; ++i) r2[i] = w2[i];
+    const float b1 = __half2float(M.w[kMlpB1 + l]);
+    const float b2 = __half2float(M.w[kMlpB2 + l]);
+    const float w3 = __half2float(M.w[kMlpW3 + l]);
+    const float b3 = __half2float(M.w[kMlpB3]);
+    float mine = 0.f;
     lds_sync();
-    const float y = grp_sum(part) + __half2float(M.w[kMlpB3]);
-    return y * M.ysd + M.ymu;
+    while (lanes) {
+        const int g = (int)(__builtin_ctzll(lanes) >> 3);
+        lanes &= lanes - 1;
+        float *area = area0 + g * stride;
+        const float4 *xv = reinterpret_cast<const float4 *>(area);
+        const float4 x0 = xv[0], x1 = xv[1];
+        float a = b1;
+        a = fmaf(a1[0], x0.x, a);
+        a = fmaf(a1[1], x0.y, a);
+        a = fmaf(a1[2], x0.z, a);
+        a = fmaf(a1[3], x0.w, a);
+        a = fmaf(a1[4], x1.x, a);
+        a = fmaf(a1[5], x1.y, a);
+        a = fmaf(a1[6], x1.z, a);
+        a = fmaf(a1[7], x1.w, a);
+        area[kMlpH + l] = fast_tanh(a);
+        lds_sync();
+        const float4 *hv = reinterpret_cast<const float4 *>(area + kMlpH);
+        float acc = b2;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            float w[8];
+            unpack8(r2[i], w);
+            const float4 h0 = hv[2 * i], h1 = hv[2 * i + 1];
+            acc = fmaf(w[0], h0.x, acc);
+            acc = fmaf(w[1], h0.y, acc);
+            acc = fmaf(w[2], h0.z, acc);
+            acc = fmaf(w[3], h0.w, acc);
+            acc = fmaf(w[4], h1.x, acc);
+            acc = fmaf(w[5], h1.y, acc);
+            acc = fmaf(w[6], h1.z, acc);
+            acc = fmaf(w[7], h1.w, acc);
+        }
+        lds_sync();
+        const float y = wave_sum(w3 * fast_tanh(acc)) + b3;
+        mine = (gw == g) ? y * M.ysd + M.ymu : mine;
+    }
+    return mine;
 }
 
 }  // namespace frg

@@ -36,39 +36,6 @@ namespace frg {
 // ------------------------------------------------------------------------------------------
 // RK6(5)8M tableau (rungekutta.cu:43-122).  kA[i][m]: stage i point = Yc + s * sum_m kA[i][m] k_m.
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ double rk_a(int i, int m) {
-    // i, m are small runtime ints; resolved by selects (no memory traffic)
-    double v = 0.0;
-    v = (i == 1 && m == 0) ? 1. / 10. : v;
-    v = (i == 2 && m == 0) ? -2. / 81. : v;
-    v = (i == 2 && m == 1) ? 20. / 81. : v;
-    v = (i == 3 && m == 0) ? 615. / 1372. : v;
-    v = (i == 3 && m == 1) ? -270. / 343. : v;
-    v = (i == 3 && m == 2) ? 1053. / 1372. : v;
-    v = (i == 4 && m == 0) ? 3243. / 5500. : v;
-    v = (i == 4 && m == 1) ? -54. / 55. : v;
-    v = (i == 4 && m == 2) ? 50949. / 71500. : v;
-    v = (i == 4 && m == 3) ? 4998. / 17875. : v;
-    v = (i == 5 && m == 0) ? -26492. / 37125. : v;
-    v = (i == 5 && m == 1) ? 72. / 55. : v;
-    v = (i == 5 && m == 2) ? 2808. / 23375. : v;
-    v = (i == 5 && m == 3) ? -24206. / 37125. : v;
-    v = (i == 5 && m == 4) ? 338. / 459. : v;
-    v = (i == 6 && m == 0) ? 5561. / 2376. : v;
-    v = (i == 6 && m == 1) ? -35. / 11. : v;
-    v = (i == 6 && m == 2) ? -24117. / 31603. : v;
-    v = (i == 6 && m == 3) ? 899983. / 200772. : v;
-    v = (i == 6 && m == 4) ? -5225. / 1836. : v;
-    v = (i == 6 && m == 5) ? 3925. / 4056. : v;
-    v = (i == 7 && m == 0) ? 465467. / 266112. : v;
-    v = (i == 7 && m == 1) ? -2945. / 1232. : v;
-    v = (i == 7 && m == 2) ? -5610201. / 14158144. : v;
-    v = (i == 7 && m == 3) ? 10513573. / 3212352. : v;
-    v = (i == 7 && m == 4) ? -424325. / 205632. : v;
-    v = (i == 7 && m == 5) ? 376225. / 454272. : v;
-    return v;
-}
-// the same coefficients as a table, read with the (per-group) stage index
 __constant__ double kA[8][7] = {
     {0, 0, 0, 0, 0, 0, 0},
     {1. / 10., 0, 0, 0, 0, 0, 0},
@@ -95,9 +62,6 @@ struct GroupLds {
     cx<R> a[32];        // eval point (16) / reduced patch rows (32) / pivot row (17) / solution (16)
 };
 
-struct Slots {
-    cx<double> v[3];
-};
 
 // Y-vector index held in slot k of lane `role` (-1: the unused third slot of lane 7).
 __device__ __forceinline__ int slot_index(int role, int k) {
@@ -336,6 +300,9 @@ struct GroupSlot {
     cx<double> par[8][12];  // per lane: start values s6[0..5] and target - start d6[0..5] of its pose
     cx<double> vc[8][3];    // per lane: its slots of the current velocity Vc
 };
+static_assert(sizeof(GroupSlot) % 16 == 0 && offsetof(GroupLds<double>, a) % 16 == 0,
+              "the step model reads float4 vectors from GroupLds::a");
+static_assert(sizeof(GroupLds<double>::a) >= 128 * sizeof(float), "features (8) and hidden layer (64 at +64)");
 
 struct PathState {
     // vectors (this lane's slots)
@@ -353,8 +320,9 @@ struct PathState {
     int st_rk, st_newton_calls, st_newton_its, st_redraws, st_ivp, st_fp32;
     // learned step control: features of the last proposal, rejections in the current step
     float fprev[kMlpIn];
-    bool have_prev;
+    bool have_prev, pending;  // pending: an accepted step waits for the model's proposal
     int rej;
+    double s_classic;         // classical proposal of a pending step (the fallback)
 };
 
 // Patch draw (htpytrack.cu:55-58,147-150): Y index i uses draws base + 2i, base + 2i + 1.
@@ -557,6 +525,7 @@ __device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, 
     P.st_ivp = 0;
     P.st_fp32 = 0;
     P.have_prev = false;
+    P.pending = false;
     P.rej = 0;
 #pragma unroll
     for (int m = 0; m < 8; ++m)
@@ -679,11 +648,21 @@ __device__ __forceinline__ void step_features(int role, const PathState &P, floa
     f[7] = (float)P.rej;
 }
 
+// Step size bounds and the next RK attempt (htpytrack.cu:178-183, then the loop top).
+__device__ __forceinline__ void set_step(const TrackOpts &o, PathState &P, double s_next) {
+    P.s = s_next;
+    if (P.s > o.smax) P.s = o.smax;
+    else if (P.s < o.smin) P.s = o.smin * 0.9;
+    main_loop_top(o, P);
+}
+
 // Accepted step (htpytrack.cu:176-183).  The next step size comes from the classical rule
 // s *= scale (scale = 0.8 (tol/err)^(1/6), capped at 5, at most 1 within cautionSteps of a
 // rejection), or -- learned control, arclength steps -- from the model, with the same cap and
-// caution rule and the classical value as the fallback for a non-finite proposal.
-__device__ __forceinline__ void accept_step(int role, const Job &J, PathState &P, float *h1buf) {
+// caution rule and the classical value as the fallback for a non-finite proposal.  A learned
+// proposal is left pending (features in the group's LDS area `xarea`): the wave evaluates the
+// model for all its pending groups at once and finish_proposal completes the step.
+__device__ __forceinline__ void accept_step(int role, const Job &J, PathState &P, float *xarea) {
     const TrackOpts &o = J.o;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -708,19 +687,7 @@ __device__ __forceinline__ void accept_step(int role, const Job &J, PathState &P
     }
     P.streak++;
     if (P.streak < o.caution_steps && P.scale > 1.) P.scale = 1.;
-    double s_next = P.s * P.scale;
-    if (J.learned && use_model) {
-        const float y = mlp_eval_group(role, J.model, f, h1buf);
-        double s_ml = exp((double)y) * 0.85;
-        if (s_ml == s_ml && s_ml < 1e300) {
-            s_ml = fmin(s_ml, 5.0 * P.s);
-            if (P.streak < o.caution_steps) s_ml = fmin(s_ml, P.s);
-            s_next = s_ml;
-        }
-    }
-    P.s = s_next;
-    if (P.s > o.smax) P.s = o.smax;
-    else if (P.s < o.smin) P.s = o.smin * 0.9;
+    const double s_classic = P.s * P.scale;
     if (use_model) {
 #pragma unroll
         for (int q = 0; q < kMlpIn; ++q) P.fprev[q] = f[q];
@@ -728,10 +695,37 @@ __device__ __forceinline__ void accept_step(int role, const Job &J, PathState &P
     P.have_prev = use_model;
     P.rej = 0;
     P.step++;
-    main_loop_top(o, P);
+    if (J.learned && use_model) {
+        float fr = f[0], mu = J.model.xmu[0], sd = J.model.xsd[0];
+#pragma unroll
+        for (int q = 1; q < kMlpIn; ++q) {
+            fr = role == q ? f[q] : fr;
+            mu = role == q ? J.model.xmu[q] : mu;
+            sd = role == q ? J.model.xsd[q] : sd;
+        }
+        lds_sync();
+        xarea[role] = (fr - mu) / sd;
+        P.s_classic = s_classic;
+        P.pending = true;
+        return;
+    }
+    set_step(o, P, s_classic);
 }
 
-__device__ __forceinline__ void finish_newton(int role, const Job &J, PathState &P, float *h1buf) {
+// Completes a pending learned proposal with the model output y (log step size).
+__device__ __forceinline__ void finish_proposal(const TrackOpts &o, PathState &P, float y) {
+    double s_next = P.s_classic;
+    double s_ml = exp((double)y) * 0.85;
+    if (s_ml == s_ml && s_ml < 1e300) {
+        s_ml = fmin(s_ml, 5.0 * P.s);
+        if (P.streak < o.caution_steps) s_ml = fmin(s_ml, P.s);
+        s_next = s_ml;
+    }
+    P.pending = false;
+    set_step(o, P, s_next);
+}
+
+__device__ __forceinline__ void finish_newton(int role, const Job &J, PathState &P, float *xarea) {
     const TrackOpts &o = J.o;
     P.st_newton_calls++;
     if (P.ministep++ == o.max_ministeps) {
@@ -740,7 +734,7 @@ __device__ __forceinline__ void finish_newton(int role, const Job &J, PathState 
         return;
     }
     if (P.residual < o.newt_tol) {
-        accept_step(role, J, P, h1buf);
+        accept_step(role, J, P, xarea);
         return;
     }
     P.rej++;
@@ -975,7 +969,7 @@ __device__ __forceinline__ void micro_step(int role, const Job &J, PathState &P,
     const double tol = P.phase == PH_SHARP ? o.sharp_tol : o.newt_tol;
     if (P.its < max_its && P.residual > tol) return;  // another iteration
     if (P.phase == PH_SHARP) finish_sharp(role, o, P);
-    else finish_newton(role, J, P, reinterpret_cast<float *>(L->a));
+    else finish_newton(role, J, P, reinterpret_cast<float *>(reinterpret_cast<GroupLds<double> *>(L)->a));
 }
 
 // Next path for a persistent group: lane 0 takes a ticket from the device queue.
@@ -1000,7 +994,8 @@ __device__ __forceinline__ bool needs_fp64(int role, const TrackOpts &o, const P
 // ------------------------------------------------------------------------------------------
 // Kernels.  Static: group g of the grid tracks path g (the reference's one-thread-per-path
 // launch, frg.cu:374).  Persistent: a fixed grid; groups take paths from the queue until it
-// runs dry.
+// runs dry.  With learned step control a group without a path stays in the loop (idle) until
+// its whole wave is idle, because the model is evaluated by all 64 lanes of the wave.
 // ------------------------------------------------------------------------------------------
 template <bool kPersistent, bool kMixed>
 __global__ __launch_bounds__(256) void track_kernel(Job J) {
@@ -1009,30 +1004,52 @@ __global__ __launch_bounds__(256) void track_kernel(Job J) {
     const int grp = threadIdx.x >> 3;
     GroupSlot *S = reinterpret_cast<GroupSlot *>(smem) + grp;
     GroupLds<double> *L = &S->lds;
+    // model scratch: the LDS areas of the wave's eight groups
+    float *xarea0 = reinterpret_cast<float *>((S - (grp & 7))->lds.a);
+    constexpr int kXStride = (int)(sizeof(GroupSlot) / sizeof(float));
     PathState P;
     P.par = S->par[role];
     P.vc = S->vc[role];
     int64_t path = kPersistent ? claim_path(role, J.queue) : (int64_t)blockIdx.x * (blockDim.x >> 3) + grp;
-    if (path >= J.n_total) return;
-    init_path(role, J, P, path);
+    bool live = path < J.n_total;
+    if (!live && !J.learned) return;
+    if (live) init_path(role, J, P, path);
     for (;;) {
-        if (kMixed) {
-            const bool need64 = needs_fp64(role, J.o, P);
-            if (__any(need64)) {
-                micro_step<double>(role, J, P, L);
+        if (live) {
+            if (kMixed) {
+                const bool need64 = needs_fp64(role, J.o, P);
+                if (__any(need64)) {
+                    micro_step<double>(role, J, P, L);
+                } else {
+                    P.st_fp32++;
+                    micro_step<float>(role, J, P, reinterpret_cast<GroupLds<float> *>(L));
+                }
             } else {
-                P.st_fp32++;
-                micro_step<float>(role, J, P, reinterpret_cast<GroupLds<float> *>(L));
+                micro_step<double>(role, J, P, L);
             }
-        } else {
-            micro_step<double>(role, J, P, L);
         }
-        if (P.done) {
+        if (J.learned) {
+            const bool pend = live && P.pending;
+            const uint64_t lanes = __ballot(pend && role == 0);
+            if (lanes) {
+                const float y = mlp_eval_wave(J.model, xarea0, kXStride, lanes, grp & 7);
+                if (pend) finish_proposal(J.o, P, y);
+            }
+        }
+        if (live && P.done) {
             finish_path(role, J, P, L);
-            if (!kPersistent) break;
-            path = claim_path(role, J.queue);
-            if (path >= J.n_total) break;
-            init_path(role, J, P, path);
+            if (kPersistent) {
+                path = claim_path(role, J.queue);
+                live = path < J.n_total;
+                if (live) init_path(role, J, P, path);
+            } else {
+                live = false;
+            }
+        }
+        if (J.learned) {
+            if (!__any(live)) break;
+        } else if (!live) {
+            break;
         }
     }
 }
