@@ -518,7 +518,8 @@ extern "C" {
 
 int frg_collect_step_log(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
                          int64_t n_paths, float *records, int64_t cap, int64_t *n_records) {
-    if (!c || !fpm || n_paths < 0 || cap < 0 || (cap > 0 && !records) || !n_records) return FRG_EINVAL;
+    if (!c || !fpm || n_paths < 0 || cap < 0 || cap > INT32_MAX || (cap > 0 && !records) || !n_records)
+        return FRG_EINVAL;
     HIPCHK(hipSetDevice(c->device));
     frg::Job job{};
     int rc = upload_instances(c, 1, fpm, &gamma, &seed, &first_path_id, &n_paths, job, c->stream);
@@ -541,7 +542,16 @@ int frg_collect_step_log(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t
     const int64_t got = std::min<int64_t>((int64_t)cnt, cap);
     if (got > 0)
         HIPCHK(hipMemcpy(records, c->steplog.p, sizeof(float) * frg::kStepRecord * got, hipMemcpyDeviceToHost));
-    *n_records = got;
+    // keep the proposals whose outcome was observed (a path's last proposal and proposals
+    // followed by a landing step have none)
+    int64_t kept = 0;
+    for (int64_t i = 0; i < got; ++i) {
+        const float *r = records + i * frg::kStepRecord;
+        if (!std::isfinite(r[frg::kStepRecord - 1])) continue;
+        if (kept != i) std::memmove(records + kept * frg::kStepRecord, r, sizeof(float) * frg::kStepRecord);
+        ++kept;
+    }
+    *n_records = kept;
     return FRG_OK;
 }
 

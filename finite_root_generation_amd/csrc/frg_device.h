@@ -133,23 +133,6 @@ __device__ __forceinline__ R grp_max(R x) {
 template <typename R>
 __device__ __forceinline__ cx<R> grp_sum(cx<R> z) { return mk(grp_sum(z.re), grp_sum(z.im)); }
 
-// Arg-max over the group: larger key wins, ties go to the smaller id.  Keys must not be NaN.
-template <int CTRL, typename R>
-__device__ __forceinline__ void argmax_step(R &key, int &id) {
-    R k2 = dpp<CTRL>(key);
-    int i2 = dpp_i<CTRL>(id);
-    bool take = (k2 > key) || (k2 == key && i2 < id);
-    key = take ? k2 : key;
-    id = take ? i2 : id;
-}
-template <typename R>
-__device__ __forceinline__ int grp_argmax(R key, int id) {
-    argmax_step<0xB1>(key, id);
-    argmax_step<0x4E>(key, id);
-    argmax_step<0x141>(key, id);
-    return id;
-}
-
 // Value of lane `src` (0..7) of the group, delivered to every lane (sum with zeros elsewhere).
 template <typename R>
 __device__ __forceinline__ R grp_bcast(R x, int role, int src) { return grp_sum(role == src ? x : R(0)); }
@@ -163,6 +146,30 @@ __device__ __forceinline__ int grp_bcast_i(int v, int role, int src) {
     x = x + dpp_i<0x4E>(x);
     x = x + dpp_i<0x141>(x);
     return x;
+}
+
+// Group maximum of an unsigned word (DPP).
+__device__ __forceinline__ uint32_t grp_max_u32(uint32_t x) {
+    x = max(x, (uint32_t)dpp_i<0xB1>((int)x));
+    x = max(x, (uint32_t)dpp_i<0x4E>((int)x));
+    x = max(x, (uint32_t)dpp_i<0x141>((int)x));
+    return x;
+}
+
+// Pivot search key of a row with pivot magnitude |a|^2 = m and id (0..15) in its group: larger
+// m wins, then the smaller id.  Finite m (compared in FP32, to 2^-19 relative) beats NaN, which
+// beats a row already used as a pivot.
+template <typename R>
+__device__ __forceinline__ uint32_t pivot_key(R m, bool used, int id) {
+    const float f = (float)m;
+    const uint32_t mag = (f == f) ? ((uint32_t)__float_as_int(f) >> 4) + 2u : 1u;
+    return ((used ? 0u : mag) << 4) | (uint32_t)(15 - id);
+}
+
+// Orders this lane's LDS accesses at compiler level only (no wave-wide code motion barrier).
+__device__ __forceinline__ void lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Orders this lane's LDS accesses before and after the call at compiler level.  LDS requests of

@@ -38,14 +38,6 @@ struct StepModel {
     float xmu[kMlpIn], xsd[kMlpIn], ymu, ysd;
 };
 
-__device__ __forceinline__ float wave_sum(float v) {
-    v = grp_sum(v);
-    v += __shfl_xor(v, 8);
-    v += __shfl_xor(v, 16);
-    v += __shfl_xor(v, 32);
-    return v;
-}
-
 __device__ __forceinline__ void unpack8(uint4 q, float out[8]) {
     const uint32_t w[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
@@ -57,33 +49,33 @@ __device__ __forceinline__ void unpack8(uint4 q, float out[8]) {
 }
 
 // Wave-wide evaluation for the groups of one wave that have a pending proposal.  Lane l owns
-// hidden unit l of both layers, so its weights are one row of W1 and one row of W2 (read once
-// per call); the groups are then evaluated one after the other.  Must be called by all 64 lanes
-// with a wave-uniform `lanes` mask whose set bits are lanes 8g of the pending groups g.  Group g
-// has written its normalised features to area0[g * stride + 0..7] (floats in LDS); the same area
-// holds that group's first hidden layer at offset 64.  `gw` is this lane's group in the wave.
-// Returns the predicted log step size in the lanes of each pending group.
-__device__ __forceinline__ float mlp_eval_wave(const StepModel &M, float *area0, int stride, uint64_t lanes, int gw) {
+// hidden unit l of both layers, so its weights are one row of W1 and one row of W2, read once
+// per call.  Must be called by all 64 lanes with a wave-uniform `lanes` mask whose set bits are
+// lanes 8g of the pending groups g.  Group g has written its normalised features to
+// area0[g * stride + 0..7] (floats in LDS); the same 128-float area then holds that group's
+// first hidden layer (at +64) and the 64 output-layer terms (at +0).  `gw` is this lane's group
+// in the wave, `role` its lane in the group.  Returns the predicted log step size in the lanes of
+// each pending group.
+__device__ __forceinline__ float mlp_eval_wave(const StepModel &M, float *area0, int stride, uint64_t lanes, int gw,
+                                               int role) {
     const int l = __lane_id();
     const uint4 *w1 = reinterpret_cast<const uint4 *>(M.w + kMlpW1 + kMlpIn * l);
     const uint4 *w2 = reinterpret_cast<const uint4 *>(M.w + kMlpW2 + kMlpH * l);
-    float a1[8];
-    unpack8(w1[0], a1);
+    const uint4 q1 = w1[0];
     uint4 r2[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) r2[i] = w2[i];
     const float b1 = __half2float(M.w[kMlpB1 + l]);
     const float b2 = __half2float(M.w[kMlpB2 + l]);
     const float w3 = __half2float(M.w[kMlpW3 + l]);
-    const float b3 = __half2float(M.w[kMlpB3]);
-    float mine = 0.f;
+    float a1[8];
+    unpack8(q1, a1);
     lds_sync();
-    while (lanes) {
-        const int g = (int)(__builtin_ctzll(lanes) >> 3);
-        lanes &= lanes - 1;
-        float *area = area0 + g * stride;
-        const float4 *xv = reinterpret_cast<const float4 *>(area);
-        const float4 x0 = xv[0], x1 = xv[1];
+    // layer 1 for every pending group
+    for (uint64_t m = lanes; m; m &= m - 1) {
+        float *area = area0 + (int)(__builtin_ctzll(m) >> 3) * stride;
+        const float4 x0 = reinterpret_cast<const float4 *>(area)[0];
+        const float4 x1 = reinterpret_cast<const float4 *>(area)[1];
         float a = b1;
         a = fmaf(a1[0], x0.x, a);
         a = fmaf(a1[1], x0.y, a);
@@ -94,7 +86,11 @@ __device__ __forceinline__ float mlp_eval_wave(const StepModel &M, float *area0,
         a = fmaf(a1[6], x1.z, a);
         a = fmaf(a1[7], x1.w, a);
         area[kMlpH + l] = fast_tanh(a);
-        lds_sync();
+    }
+    lds_sync();
+    // layer 2 and the output-layer terms w3[l] tanh(.) for every pending group
+    for (uint64_t m = lanes; m; m &= m - 1) {
+        float *area = area0 + (int)(__builtin_ctzll(m) >> 3) * stride;
         const float4 *hv = reinterpret_cast<const float4 *>(area + kMlpH);
         float acc = b2;
 #pragma unroll
@@ -111,11 +107,17 @@ __device__ __forceinline__ float mlp_eval_wave(const StepModel &M, float *area0,
             acc = fmaf(w[6], h1.z, acc);
             acc = fmaf(w[7], h1.w, acc);
         }
-        lds_sync();
-        const float y = wave_sum(w3 * fast_tanh(acc)) + b3;
-        mine = (gw == g) ? y * M.ysd + M.ymu : mine;
+        area[l] = w3 * fast_tanh(acc);
     }
-    return mine;
+    lds_sync();
+    // each group sums its own 64 terms: 8 per lane, then over the group
+    const float4 *tv = reinterpret_cast<const float4 *>(area0 + gw * stride + 8 * role);
+    const float4 t0 = tv[0], t1 = tv[1];
+    float part = ((t0.x + t0.y) + (t0.z + t0.w)) + ((t1.x + t1.y) + (t1.z + t1.w));
+    part = grp_sum(part);
+    lds_sync();
+    const float y = part + __half2float(M.w[kMlpB3]);
+    return y * M.ysd + M.ymu;
 }
 
 }  // namespace frg

@@ -224,21 +224,18 @@ __device__ __forceinline__ void group_solve(int role, GroupLds<R> *L, const cx<d
     lds_sync();
 
     // -------- 4. Gauss-Jordan on the 16 x 16 system (rows 2*role, 2*role+1) --------
+    // Pivot choice per column: the largest |a|^2 among the unused rows (pivot_key packs the
+    // magnitude and the row id into one word, so the group search is three integer maxima).
+    // The pivot lane broadcasts 1/pivot and the rest of its row through LDS.
     bool usedA = false, usedB = false;
     int pcA = 0, pcB = 0;
     cx<R> invA = zero, invB = zero;
     const int idA = 2 * role, idB = 2 * role + 1;
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
-        R ka = abs2(ra[c]), kbv = abs2(rb[c]);
-        ka = (ka == ka) ? ka : R(-1);  // NaN keys lose to every number
-        kbv = (kbv == kbv) ? kbv : R(-1);
-        ka = usedA ? R(-2) : ka;
-        kbv = usedB ? R(-2) : kbv;
-        const bool bBetter = kbv > ka;
-        const int gid = grp_argmax(bBetter ? kbv : ka, bBetter ? idB : idA);
+        const uint32_t key = max(pivot_key(abs2(ra[c]), usedA, idA), pivot_key(abs2(rb[c]), usedB, idB));
+        const int gid = 15 - (int)(grp_max_u32(key) & 15u);
         const bool pA = gid == idA, pB = gid == idB;
-        // one reciprocal per step: of whichever of this lane's rows is the pivot
         const cx<R> ipiv = inv(sel(pB, rb[c], ra[c]));
         if (pA) {
             usedA = true;
@@ -258,7 +255,7 @@ __device__ __forceinline__ void group_solve(int role, GroupLds<R> *L, const cx<d
             for (int k = 0; k < 17; ++k)
                 if (k > c) L->a[k] = rb[k];
         }
-        lds_sync();
+        lds_order();
         const cx<R> ip = L->a[c];
         const cx<R> la = sel(pA, zero, ra[c] * ip);
         const cx<R> lb = sel(pB, zero, rb[c] * ip);
@@ -270,8 +267,12 @@ __device__ __forceinline__ void group_solve(int role, GroupLds<R> *L, const cx<d
                 rb[k] = fms(rb[k], lb, pk);
             }
         }
-        lds_sync();
+        // the next column's pivot lane overwrites L->a only after these reads: LDS requests of
+        // a wave complete in issue order, and the compiler keeps stores after loads of the same
+        // addresses
+        lds_order();
     }
+    lds_sync();
     // -------- 5. solution of the shared unknowns --------
     L->a[pcA] = ra[16] * invA;
     L->a[pcB] = rb[16] * invB;
@@ -319,8 +320,8 @@ struct PathState {
     int inst;
     int st_rk, st_newton_calls, st_newton_its, st_redraws, st_ivp, st_fp32;
     // learned step control: features of the last proposal, rejections in the current step
-    float fprev[kMlpIn];
-    bool have_prev, pending;  // pending: an accepted step waits for the model's proposal
+    int rec;                  // step log: row of the last proposal's record (lane 0), or -1
+    bool pending;             // an accepted step waits for the model's proposal
     int rej;
     double s_classic;         // classical proposal of a pending step (the fallback)
 };
@@ -524,7 +525,7 @@ __device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, 
     P.st_rk = P.st_newton_calls = P.st_newton_its = P.st_redraws = 0;
     P.st_ivp = 0;
     P.st_fp32 = 0;
-    P.have_prev = false;
+    P.rec = -1;
     P.pending = false;
     P.rej = 0;
 #pragma unroll
@@ -638,13 +639,14 @@ __device__ __forceinline__ void step_features(int role, const PathState &P, floa
 #pragma unroll
     for (int k = 0; k < 3; ++k) v[k] = P.vc[k];
     const double vt = fabs(grp_bcast(v[1].re, role, 7));
+    auto lg = [](double x) { return __logf(fminf(fmaxf((float)x, 1e-30f), 1e30f)); };
     f[0] = (float)t;
-    f[1] = (float)log(P.s);
-    f[2] = (float)log(fmax(P.err, 1e-30));
+    f[1] = lg(P.s);
+    f[2] = lg(P.err);
     f[3] = (float)P.its;
-    f[4] = (float)log(fmax(P.residual, 1e-30));
-    f[5] = (float)log(fmax(ynorm, 1e-300));
-    f[6] = (float)(-log(fmax(vt, 1e-300)));
+    f[4] = lg(P.residual);
+    f[5] = lg(ynorm);
+    f[6] = -lg(vt);
     f[7] = (float)P.rej;
 }
 
@@ -672,27 +674,25 @@ __device__ __forceinline__ void accept_step(int role, const Job &J, PathState &P
     const bool use_model = (J.learned || J.steplog) && !P.tstep;
     float f[kMlpIn];
     if (use_model) step_features(role, P, f);
-    if (J.steplog && P.have_prev && !P.tstep && P.err > 0.0) {
+    if (J.steplog && role == 0) {
         // outcome of the previous proposal: the step that would have met the tolerance now
-        const float target = (float)(log(P.s) + log(o.rk_tol / P.err) / 6.0);
-        if (role == 0) {
+        if (P.rec >= 0 && !P.tstep && P.err > 0.0)
+            J.steplog[(int64_t)P.rec * kStepRecord + kMlpIn] = (float)(log(P.s) + log(o.rk_tol / P.err) / 6.0);
+        P.rec = -1;
+        if (use_model) {
             const unsigned long long i = atomicAdd(J.steplog_count, 1ull);
             if ((int64_t)i < J.steplog_cap) {
                 float *rec = J.steplog + (int64_t)i * kStepRecord;
 #pragma unroll
-                for (int q = 0; q < kMlpIn; ++q) rec[q] = P.fprev[q];
-                rec[kMlpIn] = target;
+                for (int q = 0; q < kMlpIn; ++q) rec[q] = f[q];
+                rec[kMlpIn] = __int_as_float(0x7fc00000);  // no outcome yet
+                P.rec = (int)i;
             }
         }
     }
     P.streak++;
     if (P.streak < o.caution_steps && P.scale > 1.) P.scale = 1.;
     const double s_classic = P.s * P.scale;
-    if (use_model) {
-#pragma unroll
-        for (int q = 0; q < kMlpIn; ++q) P.fprev[q] = f[q];
-    }
-    P.have_prev = use_model;
     P.rej = 0;
     P.step++;
     if (J.learned && use_model) {
@@ -715,7 +715,7 @@ __device__ __forceinline__ void accept_step(int role, const Job &J, PathState &P
 // Completes a pending learned proposal with the model output y (log step size).
 __device__ __forceinline__ void finish_proposal(const TrackOpts &o, PathState &P, float y) {
     double s_next = P.s_classic;
-    double s_ml = exp((double)y) * 0.85;
+    double s_ml = (double)__expf(y) * 0.85;
     if (s_ml == s_ml && s_ml < 1e300) {
         s_ml = fmin(s_ml, 5.0 * P.s);
         if (P.streak < o.caution_steps) s_ml = fmin(s_ml, P.s);
@@ -1032,7 +1032,7 @@ __global__ __launch_bounds__(256) void track_kernel(Job J) {
             const bool pend = live && P.pending;
             const uint64_t lanes = __ballot(pend && role == 0);
             if (lanes) {
-                const float y = mlp_eval_wave(J.model, xarea0, kXStride, lanes, grp & 7);
+                const float y = mlp_eval_wave(J.model, xarea0, kXStride, lanes, grp & 7, role);
                 if (pend) finish_proposal(J.o, P, y);
             }
         }

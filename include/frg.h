@@ -159,7 +159,8 @@ double frg_completeness(int64_t successes, int64_t duplicates);
 #define FRG_STEP_MODEL_WEIGHTS 4801
 #define FRG_STEP_RECORD 9
 
-/* Tracks n_paths paths (as frg_track, outputs discarded) and returns up to cap step records. */
+/* Tracks n_paths paths (as frg_track, outputs discarded) and returns the step records whose
+ * outcome was observed, at most cap (cap <= INT32_MAX); *n_records receives their number. */
 int frg_collect_step_log(frg_ctx *ctx, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
                          int64_t n_paths, float *records, int64_t cap, int64_t *n_records);
 
