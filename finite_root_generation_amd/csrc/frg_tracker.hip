@@ -294,12 +294,29 @@ __device__ __forceinline__ void group_solve(int role, GroupLds<R> *L, const cx<d
 // ------------------------------------------------------------------------------------------
 enum Phase : int { PH_INIT = 0, PH_RK = 1, PH_NEWTON = 2, PH_SHARP = 3 };
 
+// Per-path values that are the same in every lane of a group and rarely read.
+struct alignas(16) GroupScalars {
+    cx<double> gamma;
+    uint64_t pid, seed;  // RNG stream of the path
+    int64_t path;        // global path index (output row)
+    int inst;            // task instance
+    uint32_t draw;       // next patch draw
+    int rec;             // step log: row of the last proposal's record, or -1
+    // statistics (Newton calls = ministep - st_rk, tangent solves = 1 + 8 st_rk)
+    int st_rk, st_newton_its, st_redraws, st_fp32;
+    // TrackPath state read once per RK attempt or Newton call
+    double err, scale;  // last RK error estimate and its step factor
+    int code, step, ministep, streak, attempt, rej;
+};
+
 // Per-group LDS: the solve scratch plus state that is read once per solve and so lives in LDS
 // rather than in VGPRs (the kernel's occupancy is register-bound, LDS has room).
 struct GroupSlot {
     GroupLds<double> lds;   // also used as GroupLds<float> by FP32 solves
     cx<double> par[8][12];  // per lane: start values s6[0..5] and target - start d6[0..5] of its pose
     cx<double> vc[8][3];    // per lane: its slots of the current velocity Vc
+    cx<double> u[8][3];     // per lane: its slots of the projective patch u
+    GroupScalars gs;
 };
 static_assert(sizeof(GroupSlot) % 16 == 0 && offsetof(GroupLds<double>, a) % 16 == 0,
               "the step model reads float4 vectors from GroupLds::a");
@@ -307,23 +324,20 @@ static_assert(sizeof(GroupLds<double>::a) >= 128 * sizeof(float), "features (8) 
 
 struct PathState {
     // vectors (this lane's slots)
-    cx<double> Yc[3], u[3], K[8][3];  // K[0] doubles as YN, K[1] as Yn after an RK attempt
+    cx<double> Yc[3], K[8][3];  // K[0] doubles as YN, K[1] as Yn after an RK attempt
+    cx<double> *u;              // this lane's row of GroupSlot::u
     cx<double> *par;                 // this lane's row of GroupSlot::par (s6, d6)
     cx<double> *vc;                  // this lane's row of GroupSlot::vc
-    cx<double> gamma;
+    GroupScalars *gs;
     // scalars (identical in all lanes of the group)
-    double s, err, scale, residual, tgoal;
-    int phase, stage, code, attempt, its, step, ministep, streak;
+    double s, residual;
+    union {
+        double tgoal;      // Newton target t
+        double s_classic;  // classical proposal of a step waiting for the model (the fallback)
+    };
+    int phase, stage, its;
     bool tstep, done;
-    uint64_t draw, pid, seed;
-    int64_t path;
-    int inst;
-    int st_rk, st_newton_calls, st_newton_its, st_redraws, st_ivp, st_fp32;
-    // learned step control: features of the last proposal, rejections in the current step
-    int rec;                  // step log: row of the last proposal's record (lane 0), or -1
-    bool pending;             // an accepted step waits for the model's proposal
-    int rej;
-    double s_classic;         // classical proposal of a pending step (the fallback)
+    bool pending;  // learned step control: an accepted step waits for the model's proposal
 };
 
 // Patch draw (htpytrack.cu:55-58,147-150): Y index i uses draws base + 2i, base + 2i + 1.
@@ -333,7 +347,7 @@ __device__ __forceinline__ void draw_vec(int role, const PathState &P, uint64_t 
     for (int k = 0; k < 3; ++k) {
         const int yi = slot_index(role, k);
         double a, b;
-        uniform_pair(base / 2 + (uint64_t)(yi < 0 ? 0 : yi), P.pid, P.seed, a, b);
+        uniform_pair(base / 2 + (uint64_t)(yi < 0 ? 0 : yi), P.gs->pid, P.gs->seed, a, b);
         out[k] = sel(yi < 0, mk(0.0, 0.0), mk((hi - lo) * a + lo, (hi - lo) * b + lo));
     }
 }
@@ -387,7 +401,7 @@ __device__ __forceinline__ double grp_t(int role, const cx<double> Y[3]) { retur
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ cx<double> start_value(const TrackOpts &o, uint64_t blk, const PathState &P) {
     double a, b;
-    uniform_pair(blk, P.pid, P.seed, a, b);
+    uniform_pair(blk, P.gs->pid, P.gs->seed, a, b);
     if (o.start_mode == 1) {
         const double r = sqrt(-log(a)), th = 6.283185307179586 * b;
         double sn, cs;
@@ -406,7 +420,7 @@ __device__ __forceinline__ void generate_start(int role, const TrackOpts &o, Pat
     const cx<double> P0 = start_value(o, 14, P), Pc0 = start_value(o, 15, P);
     const cx<double> R = start_value(o, 15 + (uint64_t)j, P), iR = inv(R);
     double pmA, pmB;
-    uniform_pair(22 + (uint64_t)j, P.pid, P.seed, pmA, pmB);
+    uniform_pair(22 + (uint64_t)j, P.gs->pid, P.gs->seed, pmA, pmB);
     const cx<double> A = z[0], B = z[1], C = z[2], D = z[3], F = z[4], G = z[5], H = z[6];
     const cx<double> Ac = z[7], Bc = z[8], Cc = z[9], Dc = z[10], Fc = z[11], Gc = z[12], Hc = z[13];
     cx<double> Aq = (G - D) * (iR * (Hc - Gc) - (Hc - Fc));
@@ -459,7 +473,7 @@ __device__ __forceinline__ void generate_start(int role, const TrackOpts &o, Pat
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, int64_t path) {
     const TrackOpts &o = J.o;
-    P.path = path;
+    P.gs->path = path;
     int inst = 0;
     if (J.n_inst > 1) {
         int lo = 0, hi = J.n_inst - 1;  // last instance whose first path <= path
@@ -470,11 +484,11 @@ __device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, 
         }
         inst = lo;
     }
-    P.inst = inst;
+    P.gs->inst = inst;
     const int64_t local = path - (J.n_inst > 1 ? J.inst_first[inst] : 0);
-    P.seed = J.seed[inst];
-    P.pid = J.pid0[inst] + (uint64_t)local;
-    P.gamma = J.gamma[inst];
+    P.gs->seed = J.seed[inst];
+    P.gs->pid = J.pid0[inst] + (uint64_t)local;
+    P.gs->gamma = J.gamma[inst];
     const int j = role < 7 ? role + 1 : 7;
     const cx<double> *f = J.fpm + (int64_t)inst * 30;
     const int pidx[6] = {0, j, 8, 8 + j, 15 + j, 22 + j};  // P0, P_j, Pc0, Pc_j, Q_j, Qc_j
@@ -506,28 +520,27 @@ __device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, 
 #pragma unroll
         for (int k = 0; k < 3; ++k) P.vc[k] = v[k];
     }
-    P.draw = 92;
+    P.gs->draw = 92;
     P.phase = PH_INIT;
     P.stage = 0;
-    P.code = 0;
+    P.gs->code = 0;
     P.done = false;
     P.s = o.sinit;
     P.tstep = false;
-    P.streak = o.caution_steps - 1;
-    P.step = 0;
-    P.ministep = 0;
-    P.err = 0.0;
-    P.scale = 1.0;
+    P.gs->streak = o.caution_steps - 1;
+    P.gs->step = 0;
+    P.gs->ministep = 0;
+    P.gs->err = 0.0;
+    P.gs->scale = 1.0;
     P.residual = 100.0;
-    P.attempt = 0;
+    P.gs->attempt = 0;
     P.its = 0;
     P.tgoal = 0.0;
-    P.st_rk = P.st_newton_calls = P.st_newton_its = P.st_redraws = 0;
-    P.st_ivp = 0;
-    P.st_fp32 = 0;
-    P.rec = -1;
+    P.gs->st_rk = P.gs->st_newton_its = P.gs->st_redraws = 0;
+    P.gs->st_fp32 = 0;
+    P.gs->rec = -1;
     P.pending = false;
-    P.rej = 0;
+    P.gs->rej = 0;
 #pragma unroll
     for (int m = 0; m < 8; ++m)
 #pragma unroll
@@ -543,8 +556,8 @@ __device__ __forceinline__ void start_rk(PathState &P) {
     P.stage = 0;
 }
 __device__ __forceinline__ void main_loop_top(const TrackOpts &o, PathState &P) {
-    if (P.step == o.max_steps) {  // :85
-        P.code = 1;
+    if (P.gs->step == o.max_steps) {  // :85
+        P.gs->code = 1;
         P.done = true;
         return;
     }
@@ -560,13 +573,13 @@ __device__ __forceinline__ void start_newton(const TrackOpts &o, PathState &P, d
     P.phase = sharp ? PH_SHARP : PH_NEWTON;
 }
 __device__ __forceinline__ void redraw_patch(int role, const TrackOpts &o, PathState &P) {
-    draw_vec(role, P, P.draw, o.patch_lo, o.patch_hi, P.u);
-    P.draw += 46;
-    P.st_redraws++;
+    draw_vec(role, P, P.gs->draw, o.patch_lo, o.patch_hi, P.u);
+    P.gs->draw += 46;
+    P.gs->st_redraws++;
     switch_patch(role, P.u, P.K[1]);  // only Yn is moved to the new patch (:151, :210)
 }
 __device__ __forceinline__ void start_sharpen(const TrackOpts &o, PathState &P) {
-    P.attempt = 0;
+    P.gs->attempt = 0;
     start_newton(o, P, 1.0, true);
 }
 // corrector entry (:133-135)
@@ -575,12 +588,12 @@ __device__ __forceinline__ void correct(int role, const TrackOpts &o, PathState 
         start_sharpen(o, P);
         return;
     }
-    P.attempt = 0;
+    P.gs->attempt = 0;
     start_newton(o, P, tn, false);
 }
 // end of an RK attempt (:94-123)
 __device__ __forceinline__ void finish_rk(int role, const TrackOpts &o, PathState &P) {
-    P.st_rk++;
+    P.gs->st_rk++;
     const double s = P.s;
     cx<double> lte[3], yn[3];
 #pragma unroll
@@ -598,25 +611,25 @@ __device__ __forceinline__ void finish_rk(int role, const TrackOpts &o, PathStat
     const double err = grp_norm(lte);
     double scale = err > 10e-17 ? 0.8 * pow(o.rk_tol / err, 1. / 6.) : 5.0;
     if (scale > 5) scale = 5;
-    P.err = err;
-    P.scale = scale;
+    P.gs->err = err;
+    P.gs->scale = scale;
 #pragma unroll
     for (int k = 0; k < 3; ++k) P.K[1][k] = yn[k];  // Yn (k2 is not used after the attempt)
-    if (P.ministep++ == o.max_ministeps) {  // :95
-        P.code = 2;
+    if (P.gs->ministep++ == o.max_ministeps) {  // :95
+        P.gs->code = 2;
         P.done = true;
         return;
     }
     if (!(err <= o.rk_tol)) {  // rejected (:97-105)
         if (P.s <= o.smin) {
-            P.code = 3;
+            P.gs->code = 3;
             P.done = true;
             return;
         }
         P.s *= scale;
         if (P.s < o.smin) P.s = o.smin * 0.9;
-        P.streak = 0;
-        P.rej++;
+        P.gs->streak = 0;
+        P.gs->rej++;
         start_rk(P);
         return;
     }
@@ -642,12 +655,12 @@ __device__ __forceinline__ void step_features(int role, const PathState &P, floa
     auto lg = [](double x) { return __logf(fminf(fmaxf((float)x, 1e-30f), 1e30f)); };
     f[0] = (float)t;
     f[1] = lg(P.s);
-    f[2] = lg(P.err);
+    f[2] = lg(P.gs->err);
     f[3] = (float)P.its;
     f[4] = lg(P.residual);
     f[5] = lg(ynorm);
     f[6] = -lg(vt);
-    f[7] = (float)P.rej;
+    f[7] = (float)P.gs->rej;
 }
 
 // Step size bounds and the next RK attempt (htpytrack.cu:178-183, then the loop top).
@@ -676,9 +689,9 @@ __device__ __forceinline__ void accept_step(int role, const Job &J, PathState &P
     if (use_model) step_features(role, P, f);
     if (J.steplog && role == 0) {
         // outcome of the previous proposal: the step that would have met the tolerance now
-        if (P.rec >= 0 && !P.tstep && P.err > 0.0)
-            J.steplog[(int64_t)P.rec * kStepRecord + kMlpIn] = (float)(log(P.s) + log(o.rk_tol / P.err) / 6.0);
-        P.rec = -1;
+        if (P.gs->rec >= 0 && !P.tstep && P.gs->err > 0.0)
+            J.steplog[(int64_t)P.gs->rec * kStepRecord + kMlpIn] = (float)(log(P.s) + log(o.rk_tol / P.gs->err) / 6.0);
+        P.gs->rec = -1;
         if (use_model) {
             const unsigned long long i = atomicAdd(J.steplog_count, 1ull);
             if ((int64_t)i < J.steplog_cap) {
@@ -686,15 +699,15 @@ __device__ __forceinline__ void accept_step(int role, const Job &J, PathState &P
 #pragma unroll
                 for (int q = 0; q < kMlpIn; ++q) rec[q] = f[q];
                 rec[kMlpIn] = __int_as_float(0x7fc00000);  // no outcome yet
-                P.rec = (int)i;
+                P.gs->rec = (int)i;
             }
         }
     }
-    P.streak++;
-    if (P.streak < o.caution_steps && P.scale > 1.) P.scale = 1.;
-    const double s_classic = P.s * P.scale;
-    P.rej = 0;
-    P.step++;
+    P.gs->streak++;
+    if (P.gs->streak < o.caution_steps && P.gs->scale > 1.) P.gs->scale = 1.;
+    const double s_classic = P.s * P.gs->scale;
+    P.gs->rej = 0;
+    P.gs->step++;
     if (J.learned && use_model) {
         float fr = f[0], mu = J.model.xmu[0], sd = J.model.xsd[0];
 #pragma unroll
@@ -718,7 +731,7 @@ __device__ __forceinline__ void finish_proposal(const TrackOpts &o, PathState &P
     double s_ml = (double)__expf(y) * 0.85;
     if (s_ml == s_ml && s_ml < 1e300) {
         s_ml = fmin(s_ml, 5.0 * P.s);
-        if (P.streak < o.caution_steps) s_ml = fmin(s_ml, P.s);
+        if (P.gs->streak < o.caution_steps) s_ml = fmin(s_ml, P.s);
         s_next = s_ml;
     }
     P.pending = false;
@@ -727,9 +740,8 @@ __device__ __forceinline__ void finish_proposal(const TrackOpts &o, PathState &P
 
 __device__ __forceinline__ void finish_newton(int role, const Job &J, PathState &P, float *xarea) {
     const TrackOpts &o = J.o;
-    P.st_newton_calls++;
-    if (P.ministep++ == o.max_ministeps) {
-        P.code = 2;
+    if (P.gs->ministep++ == o.max_ministeps) {
+        P.gs->code = 2;
         P.done = true;
         return;
     }
@@ -737,43 +749,42 @@ __device__ __forceinline__ void finish_newton(int role, const Job &J, PathState 
         accept_step(role, J, P, xarea);
         return;
     }
-    P.rej++;
+    P.gs->rej++;
     redraw_patch(role, o, P);
-    P.attempt++;
-    if (P.attempt < o.newt_attempts) {
+    P.gs->attempt++;
+    if (P.gs->attempt < o.newt_attempts) {
         start_newton(o, P, grp_t(role, P.K[1]), false);
         return;
     }
     if (P.s <= o.smin) {  // :159
-        P.code = 4;
+        P.gs->code = 4;
         P.done = true;
         return;
     }
     P.s *= 0.5;
     if (P.s < o.smin) P.s = o.smin * 0.9;
-    P.streak = 0;
+    P.gs->streak = 0;
     start_rk(P);
 }
 // end of a sharpening Newton call (:198-216)
 __device__ __forceinline__ void finish_sharp(int role, const TrackOpts &o, PathState &P) {
-    P.st_newton_calls++;
-    if (P.ministep++ == o.max_ministeps) {
-        P.code = 2;
+    if (P.gs->ministep++ == o.max_ministeps) {
+        P.gs->code = 2;
         P.done = true;
         return;
     }
     if (P.residual < o.sharp_tol) {
-        P.code = 0;
+        P.gs->code = 0;
         P.done = true;
         return;
     }
     redraw_patch(role, o, P);
-    P.attempt++;
-    if (P.attempt < o.sharp_attempts) {
+    P.gs->attempt++;
+    if (P.gs->attempt < o.sharp_attempts) {
         start_newton(o, P, 1.0, true);
         return;
     }
-    P.code = 5;
+    P.gs->code = 5;
     P.done = true;
 }
 
@@ -801,7 +812,7 @@ __device__ __forceinline__ void finish_path(int role, const Job &J, PathState &P
     cx<double> Y[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) Y[k] = sel(from_sharp, P.K[0][k], P.Yc[k]);
-    const int64_t path = P.path;
+    const int64_t path = P.gs->path;
     if (J.yfinal) {
         cx<double> *yo = J.yfinal + path * 23;
         if (role < 7) {
@@ -835,8 +846,8 @@ __device__ __forceinline__ void finish_path(int role, const Job &J, PathState &P
     for (int i = 0; i < 14; ++i) lk[i] = mk(dcell[2 * i], dcell[2 * i + 1]);
     lds_sync();
     (void)cell;
-    const cx<double> P0 = J.fpm[(int64_t)P.inst * 30 + 0], Pc0 = J.fpm[(int64_t)P.inst * 30 + 8];
-    const cx<double> Pj = J.fpm[(int64_t)P.inst * 30 + (role < 7 ? role + 1 : 7)];
+    const cx<double> P0 = J.fpm[(int64_t)P.gs->inst * 30 + 0], Pc0 = J.fpm[(int64_t)P.gs->inst * 30 + 8];
+    const cx<double> Pj = J.fpm[(int64_t)P.gs->inst * 30 + (role < 7 ? role + 1 : 7)];
     cx<double> c1[7], c2[7];
     cognate12(&lk[0], P0, c1);
     cognate12(&lk[7], Pc0, c2);
@@ -863,17 +874,17 @@ __device__ __forceinline__ void finish_path(int role, const Job &J, PathState &P
         out[14 + role] = sel(swap, Udown, zd[2]);
     }
     if (role == 0) {
-        J.err[path] = P.code;
+        J.err[path] = P.gs->code;
         if (J.stats) {
             int *st = J.stats + path * 8;
-            st[0] = P.step;
-            st[1] = P.st_rk;
-            st[2] = P.st_newton_calls;
-            st[3] = P.st_newton_its;
-            st[4] = P.ministep;
-            st[5] = P.st_redraws;
-            st[6] = P.st_ivp;
-            st[7] = P.st_fp32;
+            st[0] = P.gs->step;
+            st[1] = P.gs->st_rk;
+            st[2] = P.gs->ministep - P.gs->st_rk;
+            st[3] = P.gs->st_newton_its;
+            st[4] = P.gs->ministep;
+            st[5] = P.gs->st_redraws;
+            st[6] = 1 + 8 * P.gs->st_rk;
+            st[7] = P.gs->st_fp32;
         }
     }
 }
@@ -936,13 +947,12 @@ __device__ __forceinline__ void micro_step(int role, const Job &J, PathState &P,
             e.s[p] = P.par[p];
             e.d[p] = P.par[6 + p];
         }
-        e.gamma = P.gamma;
+        e.gamma = P.gs->gamma;
         pose_values(e, fN[0], fN[1], fN[2]);
     }
     cx<double> x[3];
-    group_solve<R>(role, L, Ye, P.u, W, newton, w_rhs, P.par, P.gamma, fN, x);
+    group_solve<R>(role, L, Ye, P.u, W, newton, w_rhs, P.par, P.gs->gamma, fN, x);
     if (!newton) {
-        P.st_ivp++;
         normalize_tangent(role, x, P.tstep && P.phase != PH_INIT);
         if (P.phase == PH_INIT) {
 #pragma unroll
@@ -964,7 +974,7 @@ __device__ __forceinline__ void micro_step(int role, const Job &J, PathState &P,
     for (int k = 0; k < 3; ++k) P.K[0][k] = P.K[0][k] + x[k];
     P.residual = grp_norm(x);
     P.its++;
-    P.st_newton_its++;
+    P.gs->st_newton_its++;
     const int max_its = P.phase == PH_SHARP ? o.sharp_its : o.newt_its;
     const double tol = P.phase == PH_SHARP ? o.sharp_tol : o.newt_tol;
     if (P.its < max_its && P.residual > tol) return;  // another iteration
@@ -1010,6 +1020,8 @@ __global__ __launch_bounds__(256) void track_kernel(Job J) {
     PathState P;
     P.par = S->par[role];
     P.vc = S->vc[role];
+    P.u = S->u[role];
+    P.gs = &S->gs;
     int64_t path = kPersistent ? claim_path(role, J.queue) : (int64_t)blockIdx.x * (blockDim.x >> 3) + grp;
     bool live = path < J.n_total;
     if (!live && !J.learned) return;
@@ -1021,7 +1033,7 @@ __global__ __launch_bounds__(256) void track_kernel(Job J) {
                 if (__any(need64)) {
                     micro_step<double>(role, J, P, L);
                 } else {
-                    P.st_fp32++;
+                    P.gs->st_fp32++;
                     micro_step<float>(role, J, P, reinterpret_cast<GroupLds<float> *>(L));
                 }
             } else {
@@ -1168,9 +1180,11 @@ __global__ __launch_bounds__(64) void debug_start_kernel(DebugStartArgs A) {
     if (p >= A.n) return;
     PathState P;
     cx<double> parbuf[12];
+    GroupScalars gsbuf;
     P.par = parbuf;
-    P.seed = A.seed;
-    P.pid = A.pid0 + (uint64_t)p;
+    P.gs = &gsbuf;
+    P.gs->seed = A.seed;
+    P.gs->pid = A.pid0 + (uint64_t)p;
     generate_start(role, A.o, P);
     cx<double> *z = A.spt + p * 21;
     cx<double> *q = A.spm + p * 30;

@@ -190,6 +190,43 @@ def test_track_generated_seeded_task(tracker):
     _compare_tracks(fpt_g, err_g, fpt_o, err_o, explain=(oo, fpm, gamma, spt, spm, 7, 0))
 
 
+def test_path_statistics_match_oracle(tracker):
+    """Per-path counters (accepted steps, RK attempts, Newton calls and iterations, ministeps,
+    patch re-draws, tangent solves) against the oracle's TrackPath counts.  The counters react
+    to rounding: the oracle itself changes them on a large share of paths when the start points
+    move by 1e-15.  So the GPU must agree with the oracle per path at least 80% as often as the
+    oracle agrees with its own perturbed run, and every total must agree within 1% (rare events:
+    within a few standard deviations of their count)."""
+    n = 512
+    fpm, gamma = tasks.seeded_task(3)
+    say(f"[gpu] path statistics n={n}")
+    fpt_g, err_g, st_g, summ = tracker.track(fpm, gamma, seed=11, n=n)
+    oo = O.default_opts()
+    spt, spm = _starts(11, n)
+    fpt_o, err_o, st_o, _ = O.homotopy_explicit(oo, fpm, gamma, spt, spm, 11, 0)
+    rng = np.random.default_rng(0)
+    z = spt * (1 + 1e-15 * (rng.normal(size=spt.shape) + 1j * rng.normal(size=spt.shape)))
+    _, _, st_p, _ = O.homotopy_explicit(oo, fpm, gamma, z, spm, 11, 0)
+    pairs = [("steps", "steps"), ("rk_attempts", "rk_attempts"), ("newton_calls", "newton_calls"),
+             ("newton_its", "newton_its"), ("ministeps", "ministeps"), ("redraws", "redraws"),
+             ("tangent_solves", "ivp_calls")]
+    same_g = np.ones(n, bool)
+    same_p = np.ones(n, bool)
+    for g, o in pairs:
+        same_g &= st_g[g] == st_o[o]
+        same_p &= st_p[o] == st_o[o]
+    tg = np.array([st_g[g].sum() for g, _ in pairs], float)
+    to = np.array([st_o[o].sum() for _, o in pairs], float)
+    say(f"    identical counters: gpu vs oracle {same_g.sum()}/{n}, perturbed oracle vs oracle "
+        f"{same_p.sum()}/{n}; totals gpu {tg.astype(int).tolist()} oracle {to.astype(int).tolist()}")
+    assert same_g.mean() >= 0.8 * same_p.mean()
+    # totals: 1%, or a few standard deviations of a count for rare events (patch re-draws)
+    assert np.all(np.abs(tg - to) <= np.maximum(0.01 * to, 3 * np.sqrt(to) + 5))
+    # the tracker reports two counters through identities of TrackPath; they hold in the oracle
+    assert np.array_equal(st_o["ivp_calls"], 1 + 8 * st_o["rk_attempts"])
+    assert np.array_equal(st_o["newton_calls"], st_o["ministeps"] - st_o["rk_attempts"])
+
+
 def test_track_real_task_singular_endpoints(tracker):
     """A real task (Pc = conj P) sends many paths to singular roots, where sharpening fails
     (code 5) and rounding decides the end point; agreement is high but not total there."""

@@ -371,3 +371,12 @@ def test_reference_fixtures_consistent():
     assert oo.max_steps == d["htpytrack.cu:maxSteps"] and oo.max_ministeps == d["htpytrack.cu:maxMinisteps"]
     assert (oo.patch_lo, oo.patch_hi) == (d["htpytrack.cu:LowerRand"], d["htpytrack.cu:UpperRand"])
     assert (oo.start_lo, oo.start_hi) == (d["frgkinematics.cu:LowerRand"], d["frgkinematics.cu:UpperRand"])
+
+
+def test_trackpath_counter_identities():
+    """Every RK attempt solves 8 tangents (plus the initial one) and every ministep is an RK
+    attempt or a Newton call: the GPU tracker reports two counters through these identities."""
+    fpm, gamma = tasks.seeded_task(3)
+    fpt, err, st, _ = O.homotopy(O.default_opts(), fpm, gamma, 11, 0, 48)
+    assert np.array_equal(st["ivp_calls"], 1 + 8 * st["rk_attempts"])
+    assert np.array_equal(st["newton_calls"], st["ministeps"] - st["rk_attempts"])
