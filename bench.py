@@ -13,6 +13,12 @@ the timed region is the tracking launch alone.  The same batch is run by
 value = chosen paths/s.  One JSON line on stdout.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--paths N] [--cpu-sample N]
+  python bench.py --traffic-probe     (for rocprofv3 --pmc runs: one chosen launch, then a plain
+                                       copy of 1 GiB whose counter readings calibrate the counters)
+
+roofline.traffic comes from profiles/track_kernel_traffic.json when that file is present: HBM
+bytes per chosen launch measured by two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of the
+traffic probe, each scaled by what the counter reads for the 1 GiB copy.
 """
 from __future__ import annotations
 
@@ -77,15 +83,15 @@ def flops_split(capi, stats, model):
     return f64, f32
 
 
-def roofline(capi, res, model):
+def roofline(capi, res, model, hbm=None):
     f64, f32 = flops_split(capi, res["stats"], model)
     sec = res["kernel_ms"] * 1e-3
     tmin = f64 / (P64_TFLOPS * 1e12) + f32 / (P32_TFLOPS * 1e12)
     achieved = (f64 + f32) / sec / 1e12
     peak = (f64 + f32) / tmin / 1e12 if tmin > 0 else P64_TFLOPS
     return {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 2), "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4), "traffic": None,
-            "flops_per_launch": {"fp64": f64, "fp32": f32}}
+            "frac": round(achieved / peak, 4), "traffic": hbm["bytes"] if hbm else None,
+            "traffic_detail": hbm, "flops_per_launch": {"fp64": f64, "fp32": f32}}
 
 
 def compare(tasks, a, b, tol=1e-8):
@@ -128,6 +134,43 @@ def train_step_model(capi, tasks):
                    "seconds": round(time.perf_counter() - t0, 2)}
 
 
+def traffic(chosen_ms):
+    """HBM bytes per chosen launch from the committed counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", "track_kernel_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return {"bytes": t["hbm_bytes_per_launch"], "fetch_bytes": t["fetch_bytes"], "write_bytes": t["write_bytes"],
+            "measured_on": t["kernel"], "source": "profiles/track_kernel_traffic.json"}
+
+
+def traffic_probe(n):
+    """One chosen launch at full size, then a 1 GiB device copy (counter calibration)."""
+    import torch
+
+    from finite_root_generation_amd import capi, tasks
+    step_model, _ = train_step_model(capi, tasks)
+    fpm, gamma = tasks.seeded_task(1)
+    dev = torch.device("cuda:0")
+    fpt = torch.empty((n, 21), dtype=torch.complex128, device=dev)
+    err = torch.empty((n,), dtype=torch.int32, device=dev)
+    st = torch.empty((n, 8), dtype=torch.int32, device=dev)
+    with capi.Tracker(0, precision=capi.PREC_FP64, schedule=capi.SCHED_PERSISTENT,
+                      step_control=capi.STEP_LEARNED) as t:
+        t.set_step_model(step_model)
+        log("[probe] chosen launch")
+        t.track_device(fpm, gamma, 2, n, fpt.data_ptr(), err.data_ptr(), st.data_ptr())
+        torch.cuda.synchronize()
+        log(f"[probe] kernel {t.last_kernel_ms():.1f} ms")
+    a = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    log("[probe] 1 GiB copy")
+    b.copy_(a)
+    torch.cuda.synchronize()
+
+
 def cpu_baseline(fpm, gamma, seed, n_sample):
     from oracle import oracle as O
     cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
@@ -150,11 +193,16 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=0, help="paths for the CPU baseline (0: ~10 s of work)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ablations", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
                                    "and not launched", "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.traffic_probe:
+        traffic_probe(args.paths)
+        return
 
     import torch
 
@@ -195,7 +243,7 @@ def main():
         "config": {"workload": "stephenson2_frg_full_size", "paths": n, "task_seed": 1, "start_seed": 2,
                    "chosen": "FP64, persistent work queue, learned step control",
                    "plain": "FP64, static, classical step control"},
-        "roofline": roofline(capi, chosen, model),
+        "roofline": roofline(capi, chosen, model, traffic(chosen["kernel_ms"])),
         "plain": {"value": round(plain_value, 1), "ms_per_step": round(plain["ms_per_step"], 2),
                   "roofline": roofline(capi, plain, model)},
         "speedup_vs_plain": round(value / plain_value, 3),
