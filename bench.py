@@ -190,7 +190,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--paths", type=int, default=1 << 20)
-    ap.add_argument("--cpu-sample", type=int, default=0, help="paths for the CPU baseline (0: ~10 s of work)")
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="paths for the CPU baseline (0: 1000 per thread, about 10 s of work)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ablations", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true")
@@ -260,7 +261,7 @@ def main():
     }
     if not args.no_cpu:
         cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
-        ns = args.cpu_sample or max(64, 250 * min(16, cores))
+        ns = args.cpu_sample or max(64, 1000 * min(16, cores))
         log(f"[bench] cpu baseline: {ns} paths")
         line["cpu_baseline"] = cpu_baseline(fpm, gamma, seed, ns)
     print(json.dumps(line), flush=True)

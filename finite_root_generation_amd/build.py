@@ -38,6 +38,17 @@ def compile_cmd(out: str = LIB) -> list:
     return cmd
 
 
+ROUNDS_SRC = os.path.join(PKG, "host", "frg_rounds.cpp")
+ROUNDS_EXE = os.path.join(PKG, "frg_rounds")
+
+
+def rounds_cmd(out: str = ROUNDS_EXE) -> list:
+    """The round driver: plain C++ over the C ABI, linked against the in-tree libfrg.so."""
+    rocm_lib = os.path.join(os.path.dirname(os.path.dirname(hipcc())), "lib")
+    return [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+            ROUNDS_SRC, "-L", PKG, "-lfrg", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath-link,{rocm_lib}", "-o", out]
+
+
 def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     cmd = compile_cmd(tmp)
@@ -52,6 +63,10 @@ def build(verbose: bool = False) -> str:
     if verbose and msgs:
         print("\n".join(msgs))
     os.replace(tmp, LIB)
+    r = subprocess.run(rounds_cmd(ROUNDS_EXE + ".tmp"), capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("building frg_rounds failed:\n" + r.stderr)
+    os.replace(ROUNDS_EXE + ".tmp", ROUNDS_EXE)
     return LIB
 
 

@@ -6,7 +6,7 @@
  * says so and why the result is the same.  Build: oracle/Makefile (gcc -O2 -ffp-contract=off,
  * so no fused multiply-adds are formed and the results do not depend on the host CPU).
  *
- * Pinned by tests/test_oracle_*.py against: Philox known-answer vectors, numpy complex
+ * Pinned by tests/test_oracle.py against: Philox known-answer vectors, numpy complex
  * arithmetic, F(start) = 0 for GenerateStart output, a transcription of the reference's expanded
  * FSYS, finite differences (JVAR, JPAR, dqdt), numpy.linalg.solve (LU solves), the RK6(5)
  * tableau's order conditions, Newton's quadratic convergence, the cognate invariant, scipy's

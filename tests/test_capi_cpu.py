@@ -98,13 +98,33 @@ def test_lambertw_and_completeness():
     assert abs(est - uniq / N) < 1e-3
 
 
-def test_create_without_gpu_fails_loudly():
+def torch_has_gpu():
     try:
         import torch
-        has_gpu = torch.cuda.is_available()
+        return torch.cuda.is_available()
     except Exception:
-        has_gpu = False
-    if has_gpu:
+        return False
+
+
+def test_create_without_gpu_fails_loudly():
+    if torch_has_gpu():
         pytest.skip("a GPU is present")
     with pytest.raises(capi.FrgError):
         capi.Tracker(0)
+
+
+def test_round_driver_is_built_and_fails_loudly_without_gpu(tmp_path):
+    """frg_rounds (the C++ round driver over the C ABI) rejects bad arguments and, with no GPU,
+    stops at frg_create without writing any file."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(capi.LIB_PATH), "frg_rounds")
+    assert os.path.exists(exe)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "nRound input error" in r.stderr
+    r = subprocess.run([exe, "1", "--bogus"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
+    if not torch_has_gpu():
+        r = subprocess.run([exe, "1", "--paths", "8", "--dir", str(tmp_path)], capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 1 and "frg_create" in r.stderr
+        assert os.listdir(tmp_path) == []

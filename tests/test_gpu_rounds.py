@@ -1,0 +1,81 @@
+"""GPU tests of the round driver (finite_root_generation_amd/frg_rounds): the reference's resume
+files fptColl / errorCodesAgr / progress (frg.cu:295-336, 467-512), a resumed run, and the same
+collection recomputed through the C ABI from the same path ids."""
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from finite_root_generation_amd import capi, tasks
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(os.path.dirname(capi.LIB_PATH), "frg_rounds")
+N = 2048
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run_rounds(d, rounds, *extra):
+    say(f"[gpu] frg_rounds {rounds} --paths {N} {' '.join(extra)}")
+    r = subprocess.run([EXE, str(rounds), "--paths", str(N), "--dir", str(d), *extra], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def expected(tracker, rounds, seed=1):
+    """The collection and codes recomputed from the same path ids (round k: ids k*N ...)."""
+    coll = np.zeros((0, 21), np.complex128)
+    codes = []
+    for k in range(rounds):
+        fpt, err, _, _ = tracker.track(tasks.REF_FPM, tasks.REF_GAMMA, seed=seed, n=N, first_path_id=k * N)
+        coll, err = capi.sort_dedupe_merge(coll, fpt, err, 1e-6)
+        codes.append(err)
+    return coll, np.concatenate(codes)
+
+
+def read_files(d):
+    coll = np.fromfile(os.path.join(d, "fptColl"), dtype=np.complex128).reshape(-1, 21)
+    codes = np.fromfile(os.path.join(d, "errorCodesAgr"), dtype=np.int32)
+    progress = open(os.path.join(d, "progress")).read()
+    return coll, codes, progress
+
+
+def test_rounds_files_and_resume(tracker, tmp_path):
+    assert os.path.exists(EXE), "frg_rounds is not built"
+    run_rounds(tmp_path, 2, "--persistent")
+    coll, codes, progress = read_files(tmp_path)
+    assert codes.shape == (2 * N,)
+    assert os.path.getsize(os.path.join(tmp_path, "fptColl")) == coll.shape[0] * 336
+    ecoll, ecodes = expected(tracker, 2)
+    assert np.array_equal(codes, ecodes)
+    assert np.array_equal(coll.view(np.float64), ecoll.view(np.float64))
+    # the collection is sorted on Re z0 and holds one copy of every code-0 root
+    assert np.all(np.diff(coll[:, 0].real) >= 0)
+    assert coll.shape[0] == int((codes == 0).sum())
+    assert "-----Progress statistics for round 1-----" in progress
+    assert "-----Progress statistics for round 2-----" in progress
+
+    # resume: one more round continues with the next path ids
+    out = run_rounds(tmp_path, 1)
+    assert "Previous rounds: 2" in out and "Round 3" in out
+    coll, codes, progress = read_files(tmp_path)
+    ecoll, ecodes = expected(tracker, 3)
+    assert np.array_equal(codes, ecodes)
+    assert np.array_equal(coll.view(np.float64), ecoll.view(np.float64))
+    blocks = progress.split("-----Progress statistics for round ")
+    assert len(blocks) == 4 and blocks[3].startswith("3-----")
+    last = blocks[3]
+    roots = int(re.search(r"---In Total---\nRoots collected:\s+(\d+)", last).group(1))
+    dups = int(re.search(r"---In Total---\n.*\nDuplicates:\s+(\d+)", last).group(1))
+    trials = int(re.search(r"---In Total---\n.*\n.*\nTrials:\s+(\d+)", last).group(1))
+    assert roots == int((codes == 0).sum()) and dups == int((codes == 6).sum()) and trials == 3 * N
+    est = float(re.search(r"Roots collected \(est\.\):\s+([0-9.eE+-]+)%", last).group(1))
+    assert abs(est - 100 * capi.completeness(roots, dups)) <= 1e-3 * max(1.0, est)
