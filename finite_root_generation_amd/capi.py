@@ -104,6 +104,12 @@ def lib() -> ct.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise FrgError(f"{LIB_PATH} is not built (run `python finite_root_generation_amd/build.py`)")
+        # torch ships its own HIP runtime under the same soname; loading it first makes this library
+        # bind to that one, so a process that uses both holds a single runtime
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ct.CDLL(LIB_PATH)
         P = ct.c_void_p
         L.frg_default_opts.argtypes = [ct.POINTER(Opts)]
