@@ -12,6 +12,11 @@ the timed region is the tracking launch alone.  The same batch is run by
   * ablations (not timed for value): persistent + classical, and mixed FP32/FP64 + persistent.
 value = chosen paths/s.  One JSON line on stdout.
 
+The line also carries "batched", the many-instance workload: 64 task instances (seeded tasks
+200..263, start seeds 300..363) with path counts log-uniform in [1,024, 262,144] (numpy seed 4),
+FP32/FP64 mixed precision, launch shape picked by the autotuner on a 65,536-path sample, all
+instances in one persistent launch replayed from a hipGraph.
+
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--paths N] [--cpu-sample N]
   python bench.py --traffic-probe     (for rocprofv3 --pmc runs: one chosen launch, then a plain
                                        copy of 1 GiB whose counter readings calibrate the counters)
@@ -171,6 +176,36 @@ def traffic_probe(n):
     torch.cuda.synchronize()
 
 
+def batched(capi, tasks):
+    """64 instances in one graph-replayed persistent launch, autotuned (see the module doc)."""
+    rng = np.random.default_rng(4)
+    counts = np.exp(rng.uniform(np.log(1024), np.log(262144), 64)).astype(np.int64)
+    fpms, gams = zip(*[tasks.seeded_task(200 + k) for k in range(64)])
+    fpms, gams = np.array(fpms), np.array(gams)
+    seeds = 300 + np.arange(64)
+    pids = np.zeros(64, np.int64)
+    with capi.Tracker(0, precision=capi.PREC_MIXED, use_graph=1) as t:
+        log("[bench] batched: autotune")
+        rows, chosen = t.autotune(fpms[0], gams[0], seed=9, n_sample=65536, first_path_id=1 << 40)
+        log(f"[bench] batched: chosen schedule/block/groups-per-CU {chosen}; warm-up")
+        t.track_batch(fpms, gams, seeds, pids, counts)
+        log("[bench] batched: timed")
+        fpt, err, st, summ = t.track_batch(fpms, gams, seeds, pids, counts)
+    n = int(counts.sum())
+    names = {capi.SCHED_STATIC: "static", capi.SCHED_PERSISTENT: "persistent"}
+    return {"instances": 64, "paths": n, "path_counts": {"min": int(counts.min()), "max": int(counts.max())},
+            "value": round(n / (summ.kernel_ms * 1e-3), 1), "unit": "paths/s", "kernel_ms": round(summ.kernel_ms, 1),
+            "pcie_inclusive_paths_per_s": round(n / (summ.total_ms * 1e-3), 1),
+            "precision": "fp32+fp64 mixed", "graph": True,
+            "autotune": {"chosen": {"schedule": names[chosen[0]], "block_size": chosen[1], "groups_per_cu": chosen[2]},
+                         "sample_paths": 65536,
+                         "candidates": [{"schedule": names[r["schedule"]], "block_size": r["block_size"],
+                                         "groups_per_cu": r["groups_per_cu"], "paths_per_s": round(r["paths_per_s"], 1)}
+                                        for r in rows]},
+            "finite_roots": int(tasks.finite_mask(fpt, err).sum()),
+            "failures": np.bincount(err, minlength=6).tolist()}
+
+
 def cpu_baseline(fpm, gamma, seed, n_sample):
     from oracle import oracle as O
     cores = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
@@ -194,6 +229,7 @@ def main():
                     help="paths for the CPU baseline (0: 1000 per thread, about 10 s of work)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ablations", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -231,6 +267,7 @@ def main():
                                "rk_attempts_per_path": round(float(r["stats"]["rk_attempts"].mean()), 3),
                                "recall_vs_plain": round(c["recall"], 6), "finite_roots": c["finite_a"],
                                "failures": np.bincount(r["err"], minlength=6).tolist()}
+    batch = None if args.no_batched else batched(capi, tasks)
     value = n / (chosen["ms_per_step"] * 1e-3)
     plain_value = n / (plain["ms_per_step"] * 1e-3)
     cmp = compare(tasks, chosen, plain)
@@ -257,6 +294,7 @@ def main():
                      "plain": np.bincount(plain["err"], minlength=6).tolist()},
         "recall_vs_plain": round(cmp["recall"], 6),
         "path_length_hist": {"edges": edges.tolist(), "counts": hist.tolist()},
+        "batched": batch,
         "cpu_baseline": None,
     }
     if not args.no_cpu:

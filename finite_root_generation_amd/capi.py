@@ -35,7 +35,7 @@ EXPORTS = ["frg_default_opts", "frg_create", "frg_destroy", "frg_set_opts", "frg
            "frg_track_explicit", "frg_track_batch", "frg_generate_starts", "frg_solve_step",
            "frg_sort_dedupe_merge", "frg_lambertw", "frg_completeness", "frg_strerror", "frg_version",
            "frg_last_kernel_ms", "frg_flops", "frg_collect_step_log", "frg_train_step_model",
-           "frg_set_step_model", "frg_step_model_grad"]
+           "frg_set_step_model", "frg_step_model_grad", "frg_get_opts", "frg_autotune"]
 STEP_MODEL_SIZE, STEP_MODEL_WEIGHTS, STEP_RECORD = 4819, 4801, 9
 
 
@@ -64,7 +64,15 @@ class Opts(ct.Structure):
         ("block_size", ct.c_int32),
         ("groups_per_cu", ct.c_int32),
         ("step_control", ct.c_int32),
+        ("use_graph", ct.c_int32),
     ]
+
+
+class TuneEntry(ct.Structure):
+    """Mirror of frg_tune_entry."""
+
+    _fields_ = [("schedule", ct.c_int32), ("block_size", ct.c_int32), ("groups_per_cu", ct.c_int32),
+                ("kernel_ms", ct.c_double), ("paths_per_s", ct.c_double)]
 
 
 class Stats(ct.Structure):
@@ -134,6 +142,9 @@ def lib() -> ct.CDLL:
         L.frg_strerror.restype = ct.c_char_p
         L.frg_version.restype = ct.c_int
         L.frg_last_kernel_ms.argtypes = [P, ct.POINTER(ct.c_double)]
+        L.frg_get_opts.argtypes = [P, ct.POINTER(Opts)]
+        L.frg_autotune.argtypes = [P, P, C64, ct.c_uint64, ct.c_uint64, ct.c_int64, P, ct.c_int32,
+                                   ct.POINTER(ct.c_int32)]
         L.frg_flops.argtypes = [ct.POINTER(FlopModel)]
         L.frg_collect_step_log.argtypes = [P, P, C64, ct.c_uint64, ct.c_uint64, ct.c_int64, P, ct.c_int64,
                                            ct.POINTER(ct.c_int64)]
@@ -261,6 +272,20 @@ class Tracker:
         v = ct.c_double()
         _check(lib().frg_last_kernel_ms(self._h, ct.byref(v)), "frg_last_kernel_ms")
         return v.value
+
+    def autotune(self, fpm, gamma, seed: int, n_sample: int, first_path_id: int = 0):
+        """Measures the candidate launch shapes and keeps the fastest; returns the measurements
+        and the chosen (schedule, block_size, groups_per_cu)."""
+        fpm = _c(fpm, (NP,))
+        table = (TuneEntry * 64)()
+        k = ct.c_int32()
+        _check(lib().frg_autotune(self._h, _p(fpm), _c64(gamma), seed, first_path_id, n_sample, table, 64,
+                                  ct.byref(k)), "frg_autotune")
+        rows = [{f: getattr(table[i], f) for f, _ in TuneEntry._fields_} for i in range(k.value)]
+        o = Opts()
+        _check(lib().frg_get_opts(self._h, ct.byref(o)), "frg_get_opts")
+        self.opts = o
+        return rows, (o.schedule, o.block_size, o.groups_per_cu)
 
     def collect_step_log(self, fpm, gamma, seed: int, n: int, cap: int, first_path_id: int = 0):
         fpm = _c(fpm, (NP,))

@@ -57,6 +57,14 @@ struct frg_ctx {
     frg::StepModel model{};
     // training state
     DevBuf tw, tm, tv, tgrad, tsamples, tnorm, tloss, tidx, tw16;
+    // cached graph of the launch sequence (queue reset + tracker) and what it was captured for
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    struct GraphKey {
+        frg::Job job;
+        int kind, block, grid;
+    } graph_key;
+    bool graph_valid = false;
 };
 
 namespace {
@@ -94,8 +102,17 @@ frg::TrackOpts to_track_opts(const frg_opts &o) {
     return t;
 }
 
+void drop_graph(frg_ctx *c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    c->graph_valid = false;
+}
+
 int check_opts(const frg_opts &o) {
     if (o.newt_residual != 1) return FRG_EUNSUPPORTED;
+    if (o.use_graph != 0 && o.use_graph != 1) return FRG_EINVAL;
     if (o.start_mode != FRG_START_UNIFORM && o.start_mode != FRG_START_GAUSSIAN) return FRG_EINVAL;
     if (o.precision != FRG_PREC_FP64 && o.precision != FRG_PREC_MIXED) return FRG_EINVAL;
     if (o.schedule != FRG_SCHED_STATIC && o.schedule != FRG_SCHED_PERSISTENT) return FRG_EINVAL;
@@ -128,16 +145,46 @@ int run_job(frg_ctx *c, frg::Job &job, hipStream_t stream, frg_stats *summary) {
         if ((int64_t)grid > need) grid = (int)need;
         HIPCHK(c->queue.ensure(sizeof(unsigned long long)));
         job.queue = c->queue.as<unsigned long long>();
-        HIPCHK(frg::launch_reset_counter(job.queue, stream));
     } else {
         const int64_t g = (job.n_total + gpb - 1) / gpb;
         if (g > 0x7fffffff) return FRG_EINVAL;
         grid = (int)g;
     }
     if (job.n_total == 0) return FRG_OK;
-    HIPCHK(hipEventRecord(c->ev0, stream));
-    HIPCHK(frg::launch_track(job, kind, block, grid, stream));
-    HIPCHK(hipEventRecord(c->ev1, stream));
+    if (c->opts.use_graph) {
+        // one graph per launch signature: the kernel arguments (the Job) and the launch shape
+        frg_ctx::GraphKey key;
+        std::memset(&key, 0, sizeof(key));
+        key.job = job;
+        key.kind = kind;
+        key.block = block;
+        key.grid = grid;
+        if (!c->graph_valid || std::memcmp(&key, &c->graph_key, sizeof(key)) != 0) {
+            drop_graph(c);
+            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipSuccess;
+            if (kind == frg::LK_PERSISTENT) e = frg::launch_reset_counter(job.queue, stream);
+            if (e == hipSuccess) e = frg::launch_track(job, kind, block, grid, stream);
+            hipGraph_t g = nullptr;
+            const hipError_t e2 = hipStreamEndCapture(stream, &g);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return FRG_EHIP;
+            }
+            c->graph = g;
+            HIPCHK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+            std::memcpy(&c->graph_key, &key, sizeof(key));
+            c->graph_valid = true;
+        }
+        HIPCHK(hipEventRecord(c->ev0, stream));
+        HIPCHK(hipGraphLaunch(c->graph_exec, stream));
+        HIPCHK(hipEventRecord(c->ev1, stream));
+    } else {
+        if (kind == frg::LK_PERSISTENT) HIPCHK(frg::launch_reset_counter(job.queue, stream));
+        HIPCHK(hipEventRecord(c->ev0, stream));
+        HIPCHK(frg::launch_track(job, kind, block, grid, stream));
+        HIPCHK(hipEventRecord(c->ev1, stream));
+    }
     if (summary) {
         summary->block_size = block;
         summary->grid_size = grid;
@@ -293,6 +340,7 @@ int frg_default_opts(frg_opts *o) {
     o->block_size = 0;
     o->groups_per_cu = 0;
     o->step_control = FRG_STEP_CLASSIC;
+    o->use_graph = 0;
     return FRG_OK;
 }
 
@@ -329,6 +377,7 @@ void frg_destroy(frg_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
     DevBuf *bufs[] = {&c->inst_first, &c->fpm, &c->gamma, &c->seed, &c->pid0, &c->spt, &c->spm, &c->fpt, &c->err,
                       &c->stats, &c->yfinal, &c->queue, &c->dbg_a, &c->dbg_b, &c->dbg_c, &c->dbg_d, &c->dbg_e,
                       &c->dbg_x, &c->model16, &c->steplog, &c->steplog_count, &c->tw, &c->tm, &c->tv, &c->tgrad,
@@ -345,6 +394,13 @@ int frg_set_opts(frg_ctx *c, const frg_opts *o) {
     int rc = check_opts(*o);
     if (rc != FRG_OK) return rc;
     c->opts = *o;
+    drop_graph(c);
+    return FRG_OK;
+}
+
+int frg_get_opts(const frg_ctx *c, frg_opts *o) {
+    if (!c || !o) return FRG_EINVAL;
+    *o = c->opts;
     return FRG_OK;
 }
 
@@ -405,6 +461,68 @@ int frg_track_batch(frg_ctx *c, int32_t n_inst, const frg_c64 *fpm, const frg_c6
     if (rc != FRG_OK) return rc;
     if (job.n_total > 0 && (!fpt || !err)) return FRG_EINVAL;
     return run_host(c, job, fpt, err, path_stats, nullptr, summary, t0);
+}
+
+int frg_autotune(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
+                 int64_t n_sample, frg_tune_entry *table, int32_t table_cap, int32_t *n_entries) {
+    if (!c || !fpm || n_sample <= 0 || table_cap < 0 || (table_cap > 0 && !table)) return FRG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    const frg_opts saved = c->opts;
+    frg::Job job{};
+    int rc = upload_instances(c, 1, fpm, &gamma, &seed, &first_path_id, &n_sample, job, c->stream);
+    if (rc != FRG_OK) return rc;
+    HIPCHK(c->fpt.ensure(sizeof(frg_c64) * 21 * n_sample));
+    HIPCHK(c->err.ensure(sizeof(int32_t) * n_sample));
+    job.fpt = c->fpt.as<cx<double>>();
+    job.err = c->err.as<int>();
+    job.stats = nullptr;
+    job.yfinal = nullptr;
+    struct Cand {
+        int schedule, block, gpc;
+    };
+    std::vector<Cand> cands;
+    for (int block : {64, 128, 256}) cands.push_back({FRG_SCHED_STATIC, block, 0});
+    for (int block : {64, 128, 256}) {
+        const int gpb = block / 8;
+        const int max_gpc = frg::track_max_groups_per_cu(block, frg::LK_PERSISTENT, saved.precision);
+        for (int bpc = std::max(1, max_gpc / gpb); bpc >= 1; --bpc) cands.push_back({FRG_SCHED_PERSISTENT, block, bpc * gpb});
+    }
+    int best = -1;
+    double best_ms = 0.0;
+    int32_t k = 0;
+    c->opts.use_graph = 0;
+    for (size_t i = 0; i < cands.size(); ++i) {
+        c->opts.schedule = cands[i].schedule;
+        c->opts.block_size = cands[i].block;
+        c->opts.groups_per_cu = cands[i].gpc;
+        rc = run_job(c, job, c->stream, nullptr);
+        if (rc != FRG_OK) {
+            c->opts = saved;
+            return rc;
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        if (k < table_cap) {
+            table[k].schedule = cands[i].schedule;
+            table[k].block_size = cands[i].block;
+            table[k].groups_per_cu = cands[i].gpc;
+            table[k].kernel_ms = ms;
+            table[k].paths_per_s = ms > 0.f ? n_sample / (ms * 1e-3) : 0.0;
+            ++k;
+        }
+        if (best < 0 || ms < best_ms) {
+            best = (int)i;
+            best_ms = ms;
+        }
+    }
+    c->opts = saved;
+    c->opts.schedule = cands[best].schedule;
+    c->opts.block_size = cands[best].block;
+    c->opts.groups_per_cu = cands[best].gpc;
+    drop_graph(c);
+    if (n_entries) *n_entries = k;
+    return FRG_OK;
 }
 
 int frg_last_kernel_ms(frg_ctx *c, double *ms) {

@@ -67,6 +67,7 @@ typedef struct {
     int32_t block_size;                    /* threads per block (64..256, multiple of 64); 0 = tuned */
     int32_t groups_per_cu;                 /* persistent grid: paths in flight per CU; 0 = occupancy */
     int32_t step_control;                  /* FRG_STEP_* */
+    int32_t use_graph;                     /* 1: launch through a cached hipGraph (queue reset + tracker) */
 } frg_opts;
 
 /* Per-call summary. */
@@ -94,6 +95,9 @@ void frg_destroy(frg_ctx *ctx);
 
 /* Replaces the opts of an existing context. */
 int frg_set_opts(frg_ctx *ctx, const frg_opts *opts);
+
+/* Current opts of a context (after frg_autotune: the chosen launch shape). */
+int frg_get_opts(const frg_ctx *ctx, frg_opts *opts);
 
 /* One round: n_paths paths, path i with RNG id first_path_id + i, random start systems
  * (GenerateStart, frgkinematics.cu:13-136), gamma trick to the target fpm.  Host buffers:
@@ -123,6 +127,22 @@ int frg_track_explicit(frg_ctx *ctx, const frg_c64 *fpm, frg_c64 gamma, const fr
 int frg_track_batch(frg_ctx *ctx, int32_t n_inst, const frg_c64 *fpm, const frg_c64 *gamma, const uint64_t *seeds,
                     const uint64_t *first_path_ids, const int64_t *paths_per_inst, frg_c64 *fpt, int32_t *err,
                     int32_t *path_stats, frg_stats *summary);
+
+/* One measured launch shape of frg_autotune. */
+typedef struct {
+    int32_t schedule, block_size, groups_per_cu; /* groups_per_cu: resident paths per CU (persistent) */
+    double kernel_ms, paths_per_s;
+} frg_tune_entry;
+
+/* Times the tracker on n_sample paths (RNG ids first_path_id + i of `seed`, target fpm/gamma) for
+ * each candidate launch shape -- static or persistent schedule, 64/128/256-thread blocks, and for
+ * the persistent grid 1..max blocks per CU -- with the context's precision and step control, then
+ * sets the context's schedule, block_size and groups_per_cu to the fastest.  Up to table_cap
+ * measurements go to table (may be NULL), their number to *n_entries.  Paths per wave are fixed at
+ * 8 by the tracker's lane mapping.  No reference counterpart (the reference launches 80 x 256,
+ * frg.cu:25-27). */
+int frg_autotune(frg_ctx *ctx, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
+                 int64_t n_sample, frg_tune_entry *table, int32_t table_cap, int32_t *n_entries);
 
 /* Start systems alone (GenerateStart, frgkinematics.cu:13-136): spt (n x 21), spm (n x 30). */
 int frg_generate_starts(frg_ctx *ctx, uint64_t seed, uint64_t first_path_id, int64_t n, frg_c64 *spt, frg_c64 *spm);

@@ -1,0 +1,68 @@
+"""GPU tests of the batched launch: many task instances in one persistent launch replayed from a
+cached hipGraph (each replay checked against an eager launch, with other GPU work between the
+replays), and the autotuner's choice of launch shape."""
+import sys
+
+import numpy as np
+import pytest
+
+from finite_root_generation_amd import capi, tasks
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _instances(counts, base=20):
+    fpms, gams = zip(*[tasks.seeded_task(base + k) for k in range(len(counts))])
+    seeds = [5 + k for k in range(len(counts))]
+    pids = [1000 * k for k in range(len(counts))]
+    return np.array(fpms), np.array(gams), seeds, pids, counts
+
+
+def _same(a, b):
+    return (np.array_equal(a[1], b[1]) and np.array_equal(a[0].view(np.float64), b[0].view(np.float64))
+            and np.array_equal(a[2], b[2]))
+
+
+def test_graph_replays_equal_eager(tracker):
+    eager = capi.Tracker(0, schedule=capi.SCHED_PERSISTENT)
+    graph = capi.Tracker(0, schedule=capi.SCHED_PERSISTENT, use_graph=1)
+    try:
+        for counts in ([300, 1000, 64], [17, 512]):
+            inst = _instances(counts)
+            say(f"[gpu] batch via graph, counts {counts}")
+            ref = eager.track_batch(*inst)
+            for rep in range(3):
+                got = graph.track_batch(*inst)
+                assert _same(got, ref), (counts, rep)
+                # other GPU work between replays
+                tracker.track(tasks.REF_FPM, tasks.REF_GAMMA, seed=rep, n=96)
+    finally:
+        eager.close()
+        graph.close()
+
+
+def test_autotune_keeps_the_fastest_shape():
+    t = capi.Tracker(0)
+    try:
+        fpm, gamma = tasks.seeded_task(21)
+        say("[gpu] autotune on 8192 paths")
+        rows, chosen = t.autotune(fpm, gamma, seed=3, n_sample=8192, first_path_id=1 << 40)
+        for r in rows:
+            say(f"    sched {r['schedule']} block {r['block_size']} groups/CU {r['groups_per_cu']}: "
+                f"{r['kernel_ms']:.1f} ms, {r['paths_per_s']:.0f} paths/s")
+        assert len(rows) >= 4
+        best = min(rows, key=lambda r: r["kernel_ms"])
+        assert chosen == (best["schedule"], best["block_size"], best["groups_per_cu"])
+        # the chosen shape tracks the same paths as the default launch
+        a = t.track(fpm, gamma, seed=9, n=256)
+        with capi.Tracker(0) as d:
+            b = d.track(fpm, gamma, seed=9, n=256)
+        assert np.array_equal(a[1], b[1])
+        assert np.array_equal(a[0].view(np.float64), b[0].view(np.float64))
+    finally:
+        t.close()
