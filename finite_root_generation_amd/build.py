@@ -18,7 +18,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libfrg.so")
 ARCH = os.environ.get("FRG_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["frg_tracker.hip", "frg_mlp.hip", "frg_capi.cpp", "frg_host.cpp", "frg_flops.cpp"]
+SOURCES = ["frg_tracker.hip", "frg_mlp.hip", "frg_dedupe.hip", "frg_capi.cpp", "frg_host.cpp", "frg_flops.cpp"]
 HEADERS = ["frg_device.h", "frg_system.h", "frg_kernels.h"]
 
 

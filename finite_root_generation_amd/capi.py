@@ -35,7 +35,7 @@ EXPORTS = ["frg_default_opts", "frg_create", "frg_destroy", "frg_set_opts", "frg
            "frg_track_explicit", "frg_track_batch", "frg_generate_starts", "frg_solve_step",
            "frg_sort_dedupe_merge", "frg_lambertw", "frg_completeness", "frg_strerror", "frg_version",
            "frg_last_kernel_ms", "frg_flops", "frg_collect_step_log", "frg_train_step_model",
-           "frg_set_step_model", "frg_step_model_grad", "frg_get_opts", "frg_autotune"]
+           "frg_set_step_model", "frg_step_model_grad", "frg_get_opts", "frg_autotune", "frg_sort_dedupe_merge_device"]
 STEP_MODEL_SIZE, STEP_MODEL_WEIGHTS, STEP_RECORD = 4819, 4801, 9
 
 
@@ -143,6 +143,8 @@ def lib() -> ct.CDLL:
         L.frg_version.restype = ct.c_int
         L.frg_last_kernel_ms.argtypes = [P, ct.POINTER(ct.c_double)]
         L.frg_get_opts.argtypes = [P, ct.POINTER(Opts)]
+        L.frg_sort_dedupe_merge_device.argtypes = [P, P, ct.c_int64, P, P, ct.c_int64, ct.c_double, P,
+                                                   ct.POINTER(ct.c_int64), P]
         L.frg_autotune.argtypes = [P, P, C64, ct.c_uint64, ct.c_uint64, ct.c_int64, P, ct.c_int32,
                                    ct.POINTER(ct.c_int32)]
         L.frg_flops.argtypes = [ct.POINTER(FlopModel)]
@@ -316,6 +318,14 @@ class Tracker:
         _check(lib().frg_step_model_grad(self._h, _p(m), _p(rec), rec.shape[0], _p(g), ct.byref(loss)),
                "frg_step_model_grad")
         return g, loss.value
+
+    def sort_dedupe_merge_device(self, d_coll: int, m: int, d_fpt: int, d_err: int, n: int, d_out: int,
+                                 tol: float = 1e-6, stream: int = 0) -> int:
+        """Device pointers in, merged length out (see frg_sort_dedupe_merge_device)."""
+        k = ct.c_int64()
+        _check(lib().frg_sort_dedupe_merge_device(self._h, d_coll, m, d_fpt, d_err, n, float(tol), d_out,
+                                                  ct.byref(k), stream or None), "frg_sort_dedupe_merge_device")
+        return k.value
 
     def generate_starts(self, seed: int, n: int, first_path_id: int = 0):
         spt = np.zeros((n, NV), np.complex128)

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/frg.h"
+#include "frg_dedupe_kernels.h"
 #include "frg_kernels.h"
 #include "frg_mlp_kernels.h"
 
@@ -52,7 +53,7 @@ struct frg_ctx {
     DevBuf inst_first, fpm, gamma, seed, pid0, spt, spm, fpt, err, stats, yfinal, queue, dbg_a, dbg_b, dbg_c,
         dbg_d, dbg_e, dbg_x;
     // step model: FP16 weights on the device, normalisation on the host
-    DevBuf model16, steplog, steplog_count;
+    DevBuf model16, steplog, steplog_count, dedupe_tmp;
     bool have_model = false;
     frg::StepModel model{};
     // training state
@@ -381,7 +382,7 @@ void frg_destroy(frg_ctx *c) {
     DevBuf *bufs[] = {&c->inst_first, &c->fpm, &c->gamma, &c->seed, &c->pid0, &c->spt, &c->spm, &c->fpt, &c->err,
                       &c->stats, &c->yfinal, &c->queue, &c->dbg_a, &c->dbg_b, &c->dbg_c, &c->dbg_d, &c->dbg_e,
                       &c->dbg_x, &c->model16, &c->steplog, &c->steplog_count, &c->tw, &c->tm, &c->tv, &c->tgrad,
-                      &c->tsamples, &c->tnorm, &c->tloss, &c->tidx, &c->tw16};
+                      &c->tsamples, &c->tnorm, &c->tloss, &c->tidx, &c->tw16, &c->dedupe_tmp};
     for (DevBuf *b : bufs) b->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -522,6 +523,23 @@ int frg_autotune(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, u
     c->opts.groups_per_cu = cands[best].gpc;
     drop_graph(c);
     if (n_entries) *n_entries = k;
+    return FRG_OK;
+}
+
+int frg_sort_dedupe_merge_device(frg_ctx *c, const frg_c64 *d_coll, int64_t m, const frg_c64 *d_fpt,
+                                 int32_t *d_err, int64_t n, double tol, frg_c64 *d_coll_out, int64_t *m_out,
+                                 void *stream) {
+    if (!c || m < 0 || n < 0 || (m > 0 && !d_coll) || (n > 0 && (!d_fpt || !d_err)) || !d_coll_out || !m_out ||
+        !(tol >= 0.0) || m + n >= ((int64_t)1 << 31))
+        return FRG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    const size_t need = frg::dedupe_temp_bytes(m, n);
+    HIPCHK(c->dedupe_tmp.ensure(need));
+    HIPCHK(frg::sort_dedupe_merge_device(reinterpret_cast<const double2 *>(d_coll), m,
+                                         reinterpret_cast<const double2 *>(d_fpt), d_err, n, tol,
+                                         reinterpret_cast<double2 *>(d_coll_out), m_out, c->dedupe_tmp.p,
+                                         c->dedupe_tmp.bytes, s));
     return FRG_OK;
 }
 

@@ -166,6 +166,14 @@ int frg_solve_step(frg_ctx *ctx, int32_t mode, int32_t precision, int64_t n, con
  * 1 + r W0(-e^{-1/r}/r) with r = successes/(successes + duplicates) (frg.cu:464-465). */
 int64_t frg_sort_dedupe_merge(const frg_c64 *coll, int64_t m, const frg_c64 *fpt, int32_t *err, int64_t n,
                               double tol, frg_c64 *coll_out);
+/* frg_sort_dedupe_merge on the device, with the same rule and a bit-identical result: all four
+ * arrays are device pointers (d_coll_out: room for m + n roots), m + n < 2^31, *m_out receives
+ * the merged length.  Replaces the device sort of frg.cu:419-423 (thrust::sort_by_key) and the
+ * host SortingDeletingMerging (frgsupport.cu:10-215).  `stream` is a hipStream_t (NULL: the
+ * context's stream); the call synchronises it before returning. */
+int frg_sort_dedupe_merge_device(frg_ctx *ctx, const frg_c64 *d_coll, int64_t m, const frg_c64 *d_fpt,
+                                 int32_t *d_err, int64_t n, double tol, frg_c64 *d_coll_out, int64_t *m_out,
+                                 void *stream);
 double frg_lambertw(double z);
 double frg_completeness(int64_t successes, int64_t duplicates);
 
