@@ -380,3 +380,33 @@ def test_trackpath_counter_identities():
     fpt, err, st, _ = O.homotopy(O.default_opts(), fpm, gamma, 11, 0, 48)
     assert np.array_equal(st["ivp_calls"], 1 + 8 * st["rk_attempts"])
     assert np.array_equal(st["newton_calls"], st["ministeps"] - st["rk_attempts"])
+
+
+def test_reference_cognate34_is_not_a_root_map():
+    """The reference leaves Cognate34 (frgkinematics.cu:140-182) and the 1-2-3-4 cognate disabled
+    (frgkinematics.cu:244-266).  Restated here with the U values its commented-out code assigns
+    (U of the 3/4 cognate = U of the root), it does not map roots of the target to roots, while
+    the 1/2 cognate does -- so this build keeps it disabled too."""
+    fpm, gamma = tasks.REF_FPM, tasks.REF_GAMMA
+    fpt, err, _, _ = O.homotopy(O.default_opts(), fpm, gamma, 1, 0, 24, nthreads=4)
+
+    def c34(o):
+        A, B, C, D, F, G, H = o
+        s = (H - G) / (C - G)
+        Ap, Bp, Cp, Gp = s * (A - D) + D, s * (B - D) + D, s * (C - D) + D, D + (H - G)
+        s2 = (B - F) / (Bp - F)
+        return np.array([s2 * (Ap - F) + F, B, s2 * (Cp - F) + F, s2 * (D - F) + F, F, s2 * (Gp - F) + F,
+                         s2 * (H - F) + F])
+
+    res12, res34 = [], []
+    for i in np.nonzero(err == 0)[0]:
+        z = fpt[i]
+        z34 = np.concatenate([c34(z[0:7]), c34(z[7:14]), z[14:21]])
+        scale = lambda v: (1 + np.max(np.abs(v))) ** 3
+        res34.append(np.max(np.abs(O.fsys(np.concatenate([z34, [1.0]]), fpm))) / scale(z34))
+        pair = O.cognate_solns(z, fpm)
+        zc = pair[1] if np.array_equal(pair[0], z) else pair[0]
+        res12.append(np.max(np.abs(O.fsys(np.concatenate([zc, [1.0]]), fpm))) / scale(zc))
+    assert len(res12) >= 16
+    assert np.max(res12) < 1e-12
+    assert np.median(res34) > 1e-4
