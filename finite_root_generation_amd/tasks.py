@@ -4,9 +4,10 @@ Parameter layout (htpykinematics.cu:29-33): [P0..P7, Pc0..Pc7, Q1..Q7, Qc1..Qc7]
 
 * ``REF_FPM`` / ``REF_GAMMA``: the reference's hard-coded generic target (frg.cu:149-178) and
   gamma (frg.cu:41).
-* ``seeded_task(seed)``: a real task: precision points P0..P7 uniform in the unit disk,
-  Pc = conj(P), rotations Q_j = e^{i theta_j} with Qc = conj(Q), and gamma on the unit circle,
-  all from numpy's PCG64 generator seeded with ``seed``.
+* ``seeded_task(seed, real=False)``: a generic task (P, Pc independent in the unit disk, Q, Qc
+  complex Gaussian) or, with real=True, a physical one (Pc = conj P, |Q| = 1, Qc = conj Q); gamma
+  on the unit circle; all from numpy's PCG64 generator seeded with ``seed``.
+* ``finite_mask``: which outputs count as finite roots.
 """
 from __future__ import annotations
 

@@ -1,0 +1,80 @@
+"""Parity at the full benchmark size (2^20 paths of the seeded task 1, start seed 2), where the
+oracle is too slow to run on every path: size-independent properties instead.
+  * the persistent schedule reproduces the static launch bit for bit (FP64, classical control);
+  * every finite code-0 root is a root of the target (independent numpy form of the system);
+  * every output is the canonical member of its 1/2-cognate pair (oracle CognateSolns, sample);
+  * the oracle, run on a sample of path ids, reproduces the GPU's codes and roots there."""
+import sys
+
+import numpy as np
+import pytest
+
+from finite_root_generation_amd import capi, tasks
+from oracle import oracle as O
+from tests import stephenson_np as S
+
+pytestmark = pytest.mark.gpu
+N = 1 << 20
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def full_runs():
+    fpm, gamma = tasks.seeded_task(1)
+    out = {}
+    for name, sched in (("static", capi.SCHED_STATIC), ("persistent", capi.SCHED_PERSISTENT)):
+        with capi.Tracker(0, schedule=sched) as t:
+            say(f"[gpu] full size {name} n={N}")
+            fpt, err, st, summ = t.track(fpm, gamma, seed=2, n=N)
+            say(f"    kernel {summ.kernel_ms:.0f} ms, codes {np.bincount(err, minlength=6).tolist()}")
+            out[name] = (fpt, err, st)
+    return fpm, gamma, out
+
+
+def test_persistent_reproduces_static(full_runs):
+    _, _, out = full_runs
+    a, b = out["static"], out["persistent"]
+    assert np.array_equal(a[1], b[1])
+    assert np.array_equal(a[0].view(np.float64), b[0].view(np.float64), equal_nan=True)
+    assert np.array_equal(a[2], b[2])
+
+
+def test_roots_solve_the_target(full_runs):
+    fpm, _, out = full_runs
+    fpt, err, _ = out["persistent"]
+    ok = tasks.finite_mask(fpt, err)
+    assert ok.mean() > 0.99
+    idx = np.nonzero(ok)[0]
+    worst = 0.0
+    for lo in range(0, len(idx), 1 << 17):
+        z = fpt[idx[lo:lo + (1 << 17)]]
+        F = S.fsys(np.concatenate([z, np.ones((z.shape[0], 1))], axis=1), fpm[None, :])
+        rel = np.max(np.abs(F), axis=1) / (1.0 + np.max(np.abs(z), axis=1)) ** 3
+        worst = max(worst, float(rel.max()))
+    say(f"    worst relative residual over {len(idx)} finite roots: {worst:.2e}")
+    assert worst < 1e-9
+
+
+def test_outputs_are_canonical_cognates_and_match_the_oracle(full_runs):
+    fpm, gamma, out = full_runs
+    fpt, err, _ = out["persistent"]
+    rng = np.random.default_rng(0)
+    sample = np.sort(rng.choice(N, 48, replace=False))
+    ok = tasks.finite_mask(fpt, err)
+    for i in sample[ok[sample]]:
+        pair = O.cognate_solns(fpt[i], fpm)
+        assert np.array_equal(pair[0], fpt[i]) and pair[0][0].real <= pair[1][0].real
+    oo = O.default_opts()
+    agree = 0
+    for i in sample:
+        f_o, e_o, _, _ = O.homotopy(oo, fpm, gamma, 2, int(i), 1)
+        same = e_o[0] == err[i]
+        if same and ok[i]:
+            same = np.max(np.abs(f_o[0] - fpt[i]) / np.maximum(1.0, np.abs(fpt[i]))) <= 1e-8
+        agree += bool(same)
+    say(f"    oracle agrees on {agree}/{len(sample)} sampled paths")
+    assert agree >= len(sample) - 1
