@@ -9,7 +9,8 @@ the timed region is the tracking launch alone.  The same batch is run by
             reference's launch shape);
   * chosen: FP64, persistent grid with a device work queue, learned step-size control (model
             trained before the timed region on step logs of other tasks: seeds 10, 20, 30);
-  * ablations (not timed for value): persistent + classical, and mixed FP32/FP64 + persistent.
+  * ablations (one launch each): persistent + classical, mixed FP32/FP64 + persistent + classical,
+    and mixed + persistent + learned.
 value = chosen paths/s.  One JSON line on stdout.
 
 The line also carries "batched", the many-instance workload: 64 task instances (seeded tasks
@@ -258,10 +259,12 @@ def main():
                          fpm, gamma, n, args.steps, args.warmup, seed, model=step_model)
     ablations = {}
     if not args.no_ablations:
-        for name, kw in (("persistent_classical", dict(precision=capi.PREC_FP64, schedule=capi.SCHED_PERSISTENT)),
-                         ("mixed_persistent_classical", dict(precision=capi.PREC_MIXED,
-                                                             schedule=capi.SCHED_PERSISTENT))):
-            r = run_variant(capi, torch, name, kw, fpm, gamma, n, 1, 0, seed)
+        for name, kw, mdl in (
+                ("persistent_classical", dict(precision=capi.PREC_FP64, schedule=capi.SCHED_PERSISTENT), None),
+                ("mixed_persistent_classical", dict(precision=capi.PREC_MIXED, schedule=capi.SCHED_PERSISTENT), None),
+                ("mixed_persistent_learned", dict(precision=capi.PREC_MIXED, schedule=capi.SCHED_PERSISTENT,
+                                                  step_control=capi.STEP_LEARNED), step_model)):
+            r = run_variant(capi, torch, name, kw, fpm, gamma, n, 1, 0, seed, model=mdl)
             c = compare(tasks, r, plain)
             ablations[name] = {"value": round(n / (r["ms_per_step"] * 1e-3), 1),
                                "rk_attempts_per_path": round(float(r["stats"]["rk_attempts"].mean()), 3),
