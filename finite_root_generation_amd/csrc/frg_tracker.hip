@@ -302,8 +302,8 @@ struct alignas(16) GroupScalars {
     int inst;            // task instance
     uint32_t draw;       // next patch draw
     int rec;             // step log: row of the last proposal's record, or -1
-    // statistics (Newton calls = ministep - st_rk, tangent solves = 1 + 8 st_rk)
-    int st_rk, st_newton_its, st_redraws, st_fp32;
+    // statistics (Newton calls = ministep - st_rk)
+    int st_rk, st_newton_its, st_redraws, st_fp32, st_tan;
     // TrackPath state read once per RK attempt or Newton call
     double err, scale;  // last RK error estimate and its step factor
     int code, step, ministep, streak, attempt, rej;
@@ -536,7 +536,7 @@ __device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, 
     P.gs->attempt = 0;
     P.its = 0;
     P.tgoal = 0.0;
-    P.gs->st_rk = P.gs->st_newton_its = P.gs->st_redraws = 0;
+    P.gs->st_rk = P.gs->st_newton_its = P.gs->st_redraws = P.gs->st_tan = 0;
     P.gs->st_fp32 = 0;
     P.gs->rec = -1;
     P.pending = false;
@@ -630,7 +630,10 @@ __device__ __forceinline__ void finish_rk(int role, const TrackOpts &o, PathStat
         if (P.s < o.smin) P.s = o.smin * 0.9;
         P.gs->streak = 0;
         P.gs->rej++;
-        start_rk(P);
+        // the retry starts from the same point, velocity and patch, so its first stage tangent is
+        // the rejected attempt's k1, still in K[0]: go straight to the second stage
+        P.phase = PH_RK;
+        P.stage = 1;
         return;
     }
     const double tn = grp_t(role, P.K[1]);
@@ -885,7 +888,7 @@ __device__ __forceinline__ void finish_path(int role, const Job &J, PathState &P
             st[3] = P.gs->st_newton_its;
             st[4] = P.gs->ministep;
             st[5] = P.gs->st_redraws;
-            st[6] = 1 + 8 * P.gs->st_rk;
+            st[6] = P.gs->st_tan;
             st[7] = P.gs->st_fp32;
         }
     }
@@ -955,6 +958,7 @@ __device__ __forceinline__ void micro_step(int role, const Job &J, PathState &P,
     cx<double> x[3];
     group_solve<R>(role, L, Ye, P.u, W, newton, w_rhs, P.par, P.gs->gamma, fN, x);
     if (!newton) {
+        P.gs->st_tan++;
         normalize_tangent(role, x, P.tstep && P.phase != PH_INIT);
         if (P.phase == PH_INIT) {
 #pragma unroll

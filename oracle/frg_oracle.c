@@ -768,6 +768,7 @@ int oracle_track_path(const otrk_opts *o, const ocplx *spt, const ocplx *spm, co
                     if (ministep++ == o->max_ministeps) { code = 2; goto done; }
                     if (err <= o->rk_tol) break;
                     if (s <= o->smin) { code = 3; goto done; }
+                    S.rejections++;
                     s *= scale;
                     if (s < o->smin) s = o->smin * 0.9;
                     streak = 0;

@@ -60,6 +60,7 @@ typedef struct {
     int32_t redraws;       /* patch re-draws */
     int32_t ivp_calls;     /* IVP calls (8 per RK attempt + 1) */
     int32_t end_code;      /* error code, duplicated for convenience */
+    int32_t rejections;    /* rejected RK attempts followed by a retry */
 } otrk_stats;
 
 void oracle_default_opts(otrk_opts *o);

@@ -42,7 +42,7 @@ class Opts(ct.Structure):
 
 
 STATS_FIELDS = ["steps", "rk_attempts", "newton_calls", "newton_its", "ministeps", "redraws",
-                "ivp_calls", "end_code"]
+                "ivp_calls", "end_code", "rejections"]
 STATS_DTYPE = np.dtype([(f, np.int32) for f in STATS_FIELDS])
 
 _LIBS: dict = {}
