@@ -102,8 +102,9 @@ int frg_get_opts(const frg_ctx *ctx, frg_opts *opts);
 /* One round: n_paths paths, path i with RNG id first_path_id + i, random start systems
  * (GenerateStart, frgkinematics.cu:13-136), gamma trick to the target fpm.  Host buffers:
  * fpt (n x 21), err (n), path_stats (n x 8 int32: steps, RK attempts, Newton calls, Newton
- * iterations, ministeps, patch re-draws, tangent solves, solves done in FP32; may be NULL), summary (may be
- * NULL).  Replaces frg.cu:356-404 (InitRandStates + Homotopy + copies). */
+ * iterations, ministeps, patch re-draws, tangent solves, solves done in FP32; may be NULL),
+ * summary (may be NULL).  Tangent solves are the reference's IVP calls less one per retried
+ * rejection: a retry reuses the rejected attempt's first stage tangent, which is bit-identical.  Replaces frg.cu:356-404 (InitRandStates + Homotopy + copies). */
 int frg_track(frg_ctx *ctx, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
               int64_t n_paths, frg_c64 *fpt, int32_t *err, int32_t *path_stats, frg_stats *summary);
 
