@@ -234,8 +234,8 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
-                                   "and not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "bench.py measures one GPU; more GPUs would each track a contiguous "
+                                   "range of path ids, which is not launched here", "n_gpus": args.gpus}))
         sys.exit(2)
 
     if args.traffic_probe:
