@@ -5,7 +5,7 @@
  * routines (mplecnik/Finite_Root_Generation: classes.cuh, support.cu, htpykinematics.cu,
  * htpysupport.cu, rungekutta.cu, htpytrack.cu, frgkinematics.cu, the Homotopy kernel body
  * frg.cu:35-69, and the host post-processing in frgsupport.cu).  Only tests/, the smoke check
- * and the CPU-baseline leg of bench.py may load it, and only as the checker: the product path
+ * and the CPU-baseline leg of bench.py may load it, and only to compare against: the product path
  * (finite_root_generation_amd/) never links or calls it.
  *
  * Random numbers: the reference draws from curand XORWOW, which is not reproduced.  Both this

@@ -386,7 +386,7 @@ def test_reference_cognate34_is_not_a_root_map():
     """The reference leaves Cognate34 (frgkinematics.cu:140-182) and the 1-2-3-4 cognate disabled
     (frgkinematics.cu:244-266).  Restated here with the U values its commented-out code assigns
     (U of the 3/4 cognate = U of the root), it does not map roots of the target to roots, while
-    the 1/2 cognate does -- so this build keeps it disabled too."""
+    the 1/2 cognate does -- so the tracker keeps it disabled too."""
     fpm, gamma = tasks.REF_FPM, tasks.REF_GAMMA
     fpt, err, _, _ = O.homotopy(O.default_opts(), fpm, gamma, 1, 0, 24, nthreads=4)
 
