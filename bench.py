@@ -15,8 +15,10 @@ value = chosen paths/s.  One JSON line on stdout.
 
 The line also carries "batched", the many-instance workload: 64 task instances (seeded tasks
 200..263, start seeds 300..363) with path counts log-uniform in [1,024, 262,144] (numpy seed 4),
-FP32/FP64 mixed precision, launch shape picked by the autotuner on a 65,536-path sample, all
-instances in one persistent launch replayed from a hipGraph.
+FP32/FP64 mixed precision and learned step control, launch shape picked by the autotuner on a
+65,536-path sample, all instances in one persistent launch replayed from a hipGraph; next to it the
+plain way, each instance its own static FP64 launch with classical control, as the reference runs
+one target at a time.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--paths N] [--cpu-sample N]
   python bench.py --traffic-probe     (for rocprofv3 --pmc runs: one chosen launch, then a plain
@@ -103,7 +105,7 @@ def roofline(capi, res, model, hbm=None):
 def compare(tasks, a, b, tol=1e-8):
     """Recall of b's finite roots by a: a root of b counts as found when a has a finite root
     within tol (max over components of |dx| / max(1, |x|)) -- on the same path, or else anywhere
-    (nearest-neighbour search over a's roots)."""
+    (candidates from a window of +-tol(1 + |Re z0|) on a's sorted Re z0, then every component)."""
     fa = tasks.finite_mask(a["fpt"], a["err"])
     fb = tasks.finite_mask(b["fpt"], b["err"])
     both = fa & fb
@@ -113,14 +115,18 @@ def compare(tasks, a, b, tol=1e-8):
     found = int(match.sum())
     rest = np.nonzero(fb & ~match)[0]
     if len(rest) and fa.any():
-        from scipy.spatial import cKDTree
-
-        def emb(x):
-            x = x / np.maximum(1.0, np.max(np.abs(x), axis=1, keepdims=True))
-            return np.concatenate([x.real, x.imag], axis=1)
-        tree = cKDTree(emb(a["fpt"][fa]))
-        dist, _ = tree.query(emb(b["fpt"][rest]), k=1, p=np.inf)
-        found += int((dist <= tol).sum())
+        ra = a["fpt"][fa]
+        order = np.argsort(ra[:, 0].real, kind="stable")
+        keys = ra[order, 0].real
+        for i in rest:
+            x = b["fpt"][i]
+            w = tol * max(1.0, abs(x[0]))
+            lo, hi = np.searchsorted(keys, [x[0].real - w, x[0].real + w], side="left")[0], \
+                np.searchsorted(keys, x[0].real + w, side="right")
+            if hi > lo:
+                c = ra[order[lo:hi]]
+                if np.any(np.max(np.abs(c - x) / np.maximum(1.0, np.abs(x)), axis=1) <= tol):
+                    found += 1
     return {"finite_a": int(fa.sum()), "finite_b": int(fb.sum()), "per_path_match": int(match.sum()),
             "recall": float(found / max(1, fb.sum()))}
 
@@ -177,15 +183,17 @@ def traffic_probe(n):
     torch.cuda.synchronize()
 
 
-def batched(capi, tasks):
-    """64 instances in one graph-replayed persistent launch, autotuned (see the module doc)."""
+def batched(capi, tasks, step_model, with_plain=True):
+    """64 instances in one graph-replayed persistent launch, autotuned (see the module doc); with
+    with_plain, also the plain way: each instance its own static FP64 launch, classical control."""
     rng = np.random.default_rng(4)
     counts = np.exp(rng.uniform(np.log(1024), np.log(262144), 64)).astype(np.int64)
     fpms, gams = zip(*[tasks.seeded_task(200 + k) for k in range(64)])
     fpms, gams = np.array(fpms), np.array(gams)
     seeds = 300 + np.arange(64)
     pids = np.zeros(64, np.int64)
-    with capi.Tracker(0, precision=capi.PREC_MIXED, use_graph=1) as t:
+    with capi.Tracker(0, precision=capi.PREC_MIXED, use_graph=1, step_control=capi.STEP_LEARNED) as t:
+        t.set_step_model(step_model)
         log("[bench] batched: autotune")
         rows, chosen = t.autotune(fpms[0], gams[0], seed=9, n_sample=65536, first_path_id=1 << 40)
         log(f"[bench] batched: chosen schedule/block/groups-per-CU {chosen}; warm-up")
@@ -193,18 +201,37 @@ def batched(capi, tasks):
         log("[bench] batched: timed")
         fpt, err, st, summ = t.track_batch(fpms, gams, seeds, pids, counts)
     n = int(counts.sum())
+    plain = None
+    if with_plain:
+        log("[bench] batched: plain, one static FP64 launch per instance")
+        plain_ms = 0.0
+        pf, pe = [], []
+        with capi.Tracker(0) as t:
+            for k in range(64):
+                f, e, _, _ = t.track(fpms[k], gams[k], int(seeds[k]), int(counts[k]), want_stats=False)
+                plain_ms += t.last_kernel_ms()
+                pf.append(f)
+                pe.append(e)
+                if k % 8 == 7:
+                    log(f"[bench] batched: plain, {k + 1}/64 instances")
+        log("[bench] batched: recall against the plain launches")
+        c = compare(tasks, {"fpt": fpt, "err": err}, {"fpt": np.concatenate(pf), "err": np.concatenate(pe)})
+        plain = {"value": round(n / (plain_ms * 1e-3), 1), "kernel_ms": round(plain_ms, 1),
+                 "how": "64 separate launches, static schedule, FP64, classical step control",
+                 "finite_roots": c["finite_b"], "recall_of_plain_roots": round(c["recall"], 6)}
     names = {capi.SCHED_STATIC: "static", capi.SCHED_PERSISTENT: "persistent"}
     return {"instances": 64, "paths": n, "path_counts": {"min": int(counts.min()), "max": int(counts.max())},
             "value": round(n / (summ.kernel_ms * 1e-3), 1), "unit": "paths/s", "kernel_ms": round(summ.kernel_ms, 1),
             "pcie_inclusive_paths_per_s": round(n / (summ.total_ms * 1e-3), 1),
-            "precision": "fp32+fp64 mixed", "graph": True,
+            "precision": "fp32+fp64 mixed", "step_control": "learned", "graph": True,
             "autotune": {"chosen": {"schedule": names[chosen[0]], "block_size": chosen[1], "groups_per_cu": chosen[2]},
                          "sample_paths": 65536,
                          "candidates": [{"schedule": names[r["schedule"]], "block_size": r["block_size"],
                                          "groups_per_cu": r["groups_per_cu"], "paths_per_s": round(r["paths_per_s"], 1)}
                                         for r in rows]},
             "finite_roots": int(tasks.finite_mask(fpt, err).sum()),
-            "failures": np.bincount(err, minlength=6).tolist()}
+            "failures": np.bincount(err, minlength=6).tolist(),
+            "plain": plain, "speedup_vs_plain": round(n / (summ.kernel_ms * 1e-3) / plain["value"], 3) if plain else None}
 
 
 def cpu_baseline(fpm, gamma, seed, n_sample):
@@ -231,6 +258,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ablations", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-batched-plain", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -270,7 +298,7 @@ def main():
                                "rk_attempts_per_path": round(float(r["stats"]["rk_attempts"].mean()), 3),
                                "recall_vs_plain": round(c["recall"], 6), "finite_roots": c["finite_a"],
                                "failures": np.bincount(r["err"], minlength=6).tolist()}
-    batch = None if args.no_batched else batched(capi, tasks)
+    batch = None if args.no_batched else batched(capi, tasks, step_model, with_plain=not args.no_batched_plain)
     value = n / (chosen["ms_per_step"] * 1e-3)
     plain_value = n / (plain["ms_per_step"] * 1e-3)
     cmp = compare(tasks, chosen, plain)
