@@ -96,9 +96,12 @@ __host__ __device__ __forceinline__ cx<double> csqrt_ref(cx<double> x) {
 // 8-lane group primitives (DPP).  DPP controls (gfx9): quad_perm [1,0,3,2] = 0xB1,
 // quad_perm [2,3,0,1] = 0x4E, row_half_mirror = 0x141 (lane i <-> 7-i within each 8 lanes).
 // ------------------------------------------------------------------------------------------
+// Every control used here reads a lane of the same 8-lane group, so no lane falls outside the
+// row and the old value and bound_ctrl never matter; old = 0 with bound_ctrl set lets the
+// compiler fold the move into the instruction that consumes it (v_max_u32_dpp and the like).
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int v) {
-    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_d(double x) {
