@@ -242,9 +242,15 @@ def cpu_baseline(fpm, gamma, seed, n_sample):
     t0 = time.perf_counter()
     O.homotopy(oo, fpm, gamma, seed, 0, n_sample, nthreads=cores, fast=True)
     dt = time.perf_counter() - t0
+    n_one = max(16, n_sample // (8 * cores))
+    t0 = time.perf_counter()
+    O.homotopy(oo, fpm, gamma, seed, 0, n_one, nthreads=1, fast=True)
+    dt1 = time.perf_counter() - t0
     return {"value": round(n_sample / dt, 2), "unit": "paths/s", "cores": cores, "kind": "port",
             "sample": f"first {n_sample} paths of the same workload (same task, seeds and RNG ids), "
-                      f"CPU oracle -O3, OpenMP {cores} threads, {dt:.1f} s"}
+                      f"CPU oracle -O3, OpenMP {cores} threads, {dt:.1f} s",
+            "single_core": {"value": round(n_one / dt1, 2), "unit": "paths/s",
+                            "sample": f"first {n_one} paths, one thread, {dt1:.1f} s"}}
 
 
 def main():
