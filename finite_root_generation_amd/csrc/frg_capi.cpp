@@ -49,6 +49,10 @@ struct frg_ctx {
     int num_cu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // recorded after work that an entry point leaves running on a caller's stream
+    // (frg_track_device, frg_sort_dedupe_merge_device); every entry point waits for it before it
+    // touches the context's buffers
+    hipEvent_t ev_busy = nullptr;
     frg_opts opts{};
     DevBuf inst_first, fpm, gamma, seed, pid0, spt, spm, fpt, err, stats, yfinal, queue, dbg_a, dbg_b, dbg_c,
         dbg_d, dbg_e, dbg_x;
@@ -74,6 +78,21 @@ namespace {
     do {                                       \
         hipError_t e_ = (x);                   \
         if (e_ != hipSuccess) return FRG_EHIP; \
+    } while (0)
+
+// Makes the context's device current and waits until no earlier work of the context (on its own
+// stream or on a caller's stream) can still read or write the context's buffers.
+int begin(frg_ctx *c) {
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipEventSynchronize(c->ev_busy));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FRG_OK;
+}
+
+#define BEGIN(c)                       \
+    do {                               \
+        const int rc_ = begin(c);      \
+        if (rc_ != FRG_OK) return rc_; \
     } while (0)
 
 frg::TrackOpts to_track_opts(const frg_opts &o) {
@@ -360,7 +379,8 @@ int frg_create(frg_ctx **out, int device, const frg_opts *opts) {
     c->device = device;
     c->opts = o;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_busy, hipEventDisableTiming) != hipSuccess) {
         frg_destroy(c);
         return FRG_EHIP;
     }
@@ -377,6 +397,7 @@ int frg_create(frg_ctx **out, int device, const frg_opts *opts) {
 void frg_destroy(frg_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
+    if (c->ev_busy) (void)hipEventSynchronize(c->ev_busy);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
     DevBuf *bufs[] = {&c->inst_first, &c->fpm, &c->gamma, &c->seed, &c->pid0, &c->spt, &c->spm, &c->fpt, &c->err,
@@ -386,6 +407,7 @@ void frg_destroy(frg_ctx *c) {
     for (DevBuf *b : bufs) b->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev_busy) (void)hipEventDestroy(c->ev_busy);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -394,6 +416,7 @@ int frg_set_opts(frg_ctx *c, const frg_opts *o) {
     if (!c || !o) return FRG_EINVAL;
     int rc = check_opts(*o);
     if (rc != FRG_OK) return rc;
+    BEGIN(c);
     c->opts = *o;
     drop_graph(c);
     return FRG_OK;
@@ -409,7 +432,7 @@ int frg_track(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint
               frg_c64 *fpt, int32_t *err, int32_t *path_stats, frg_stats *summary) {
     if (!c || !fpm || n_paths < 0 || (n_paths > 0 && (!fpt || !err))) return FRG_EINVAL;
     auto t0 = std::chrono::steady_clock::now();
-    HIPCHK(hipSetDevice(c->device));
+    BEGIN(c);
     frg::Job job{};
     int rc = upload_instances(c, 1, fpm, &gamma, &seed, &first_path_id, &n_paths, job, c->stream);
     if (rc != FRG_OK) return rc;
@@ -419,7 +442,7 @@ int frg_track(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint
 int frg_track_device(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
                      int64_t n_paths, frg_c64 *d_fpt, int32_t *d_err, int32_t *d_path_stats, void *stream) {
     if (!c || !fpm || n_paths < 0 || (n_paths > 0 && (!d_fpt || !d_err))) return FRG_EINVAL;
-    HIPCHK(hipSetDevice(c->device));
+    BEGIN(c);
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
     frg::Job job{};
     int rc = upload_instances(c, 1, fpm, &gamma, &seed, &first_path_id, &n_paths, job, s);
@@ -428,7 +451,10 @@ int frg_track_device(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t see
     job.err = d_err;
     job.stats = d_path_stats;
     job.yfinal = nullptr;
-    return run_job(c, job, s, nullptr);
+    rc = run_job(c, job, s, nullptr);
+    if (rc != FRG_OK) return rc;
+    HIPCHK(hipEventRecord(c->ev_busy, s));
+    return FRG_OK;
 }
 
 int frg_track_explicit(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, const frg_c64 *spt, const frg_c64 *spm,
@@ -436,7 +462,7 @@ int frg_track_explicit(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, const frg_
                        int32_t *path_stats, frg_c64 *yfinal, frg_stats *summary) {
     if (!c || !fpm || n_paths < 0 || (n_paths > 0 && (!spt || !spm || !fpt || !err))) return FRG_EINVAL;
     auto t0 = std::chrono::steady_clock::now();
-    HIPCHK(hipSetDevice(c->device));
+    BEGIN(c);
     frg::Job job{};
     int rc = upload_instances(c, 1, fpm, &gamma, &seed, &first_path_id, &n_paths, job, c->stream);
     if (rc != FRG_OK) return rc;
@@ -456,7 +482,7 @@ int frg_track_batch(frg_ctx *c, int32_t n_inst, const frg_c64 *fpm, const frg_c6
                     int32_t *path_stats, frg_stats *summary) {
     if (!c || n_inst <= 0 || !fpm || !gamma || !seeds || !first_path_ids || !paths_per_inst) return FRG_EINVAL;
     auto t0 = std::chrono::steady_clock::now();
-    HIPCHK(hipSetDevice(c->device));
+    BEGIN(c);
     frg::Job job{};
     int rc = upload_instances(c, n_inst, fpm, gamma, seeds, first_path_ids, paths_per_inst, job, c->stream);
     if (rc != FRG_OK) return rc;
@@ -467,7 +493,7 @@ int frg_track_batch(frg_ctx *c, int32_t n_inst, const frg_c64 *fpm, const frg_c6
 int frg_autotune(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
                  int64_t n_sample, frg_tune_entry *table, int32_t table_cap, int32_t *n_entries) {
     if (!c || !fpm || n_sample <= 0 || table_cap < 0 || (table_cap > 0 && !table)) return FRG_EINVAL;
-    HIPCHK(hipSetDevice(c->device));
+    BEGIN(c);
     const frg_opts saved = c->opts;
     frg::Job job{};
     int rc = upload_instances(c, 1, fpm, &gamma, &seed, &first_path_id, &n_sample, job, c->stream);
@@ -532,7 +558,7 @@ int frg_sort_dedupe_merge_device(frg_ctx *c, const frg_c64 *d_coll, int64_t m, c
     if (!c || m < 0 || n < 0 || (m > 0 && !d_coll) || (n > 0 && (!d_fpt || !d_err)) || !d_coll_out || !m_out ||
         !(tol >= 0.0) || m + n >= ((int64_t)1 << 31))
         return FRG_EINVAL;
-    HIPCHK(hipSetDevice(c->device));
+    BEGIN(c);
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
     const size_t need = frg::dedupe_temp_bytes(m, n);
     HIPCHK(c->dedupe_tmp.ensure(need));
@@ -540,12 +566,13 @@ int frg_sort_dedupe_merge_device(frg_ctx *c, const frg_c64 *d_coll, int64_t m, c
                                          reinterpret_cast<const double2 *>(d_fpt), d_err, n, tol,
                                          reinterpret_cast<double2 *>(d_coll_out), m_out, c->dedupe_tmp.p,
                                          c->dedupe_tmp.bytes, s));
+    HIPCHK(hipEventRecord(c->ev_busy, s));
     return FRG_OK;
 }
 
 int frg_last_kernel_ms(frg_ctx *c, double *ms) {
     if (!c || !ms) return FRG_EINVAL;
-    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipSetDevice(c->device));  // no BEGIN: reads only the events
     HIPCHK(hipEventSynchronize(c->ev1));
     float v = 0.f;
     HIPCHK(hipEventElapsedTime(&v, c->ev0, c->ev1));
@@ -556,7 +583,7 @@ int frg_last_kernel_ms(frg_ctx *c, double *ms) {
 int frg_generate_starts(frg_ctx *c, uint64_t seed, uint64_t first_path_id, int64_t n, frg_c64 *spt, frg_c64 *spm) {
     if (!c || n < 0 || (n > 0 && (!spt || !spm))) return FRG_EINVAL;
     if (n == 0) return FRG_OK;
-    HIPCHK(hipSetDevice(c->device));
+    BEGIN(c);
     HIPCHK(c->spt.ensure(sizeof(frg_c64) * 21 * n));
     HIPCHK(c->spm.ensure(sizeof(frg_c64) * 30 * n));
     frg::DebugStartArgs a{};
@@ -580,7 +607,7 @@ int frg_solve_step(frg_ctx *c, int32_t mode, int32_t precision, int64_t n, const
     if (precision != FRG_PREC_FP64 && precision != FRG_PREC_MIXED) return FRG_EINVAL;
     if (n == 0) return FRG_OK;
     if (!Y || !u || !spm || !fpm || !x || (mode < 2 && !w) || (mode == 2 && !tgoal)) return FRG_EINVAL;
-    HIPCHK(hipSetDevice(c->device));
+    BEGIN(c);
     const size_t v23 = sizeof(frg_c64) * 23 * n;
     HIPCHK(c->dbg_a.ensure(v23));
     HIPCHK(c->dbg_b.ensure(v23));
@@ -656,7 +683,7 @@ int frg_collect_step_log(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t
                          int64_t n_paths, float *records, int64_t cap, int64_t *n_records) {
     if (!c || !fpm || n_paths < 0 || cap < 0 || cap > INT32_MAX || (cap > 0 && !records) || !n_records)
         return FRG_EINVAL;
-    HIPCHK(hipSetDevice(c->device));
+    BEGIN(c);
     frg::Job job{};
     int rc = upload_instances(c, 1, fpm, &gamma, &seed, &first_path_id, &n_paths, job, c->stream);
     if (rc != FRG_OK) return rc;
@@ -698,7 +725,7 @@ int frg_set_step_model(frg_ctx *c, const float *model) {
     for (int i = 0; i < frg::kMlpIn; ++i)
         if (!(model[frg::kMlpXsd + i] > 0.f)) return FRG_EINVAL;
     if (!(model[frg::kMlpYsd] > 0.f)) return FRG_EINVAL;
-    HIPCHK(hipSetDevice(c->device));
+    BEGIN(c);
     HIPCHK(c->tw.ensure(sizeof(float) * frg::kMlpNW));
     HIPCHK(c->model16.ensure(sizeof(__half) * frg::kMlpNW));
     HIPCHK(hipMemcpyAsync(c->tw.p, model, sizeof(float) * frg::kMlpNW, hipMemcpyHostToDevice, c->stream));
@@ -712,7 +739,7 @@ int frg_set_step_model(frg_ctx *c, const float *model) {
 
 int frg_step_model_grad(frg_ctx *c, const float *model, const float *records, int64_t n, float *grad, float *loss) {
     if (!c || !model || !records || n <= 0 || !grad) return FRG_EINVAL;
-    HIPCHK(hipSetDevice(c->device));
+    BEGIN(c);
     HIPCHK(c->tw.ensure(sizeof(float) * frg::kMlpNW));
     HIPCHK(c->tgrad.ensure(sizeof(float) * frg::kMlpNW));
     HIPCHK(c->tsamples.ensure(sizeof(float) * frg::kStepRecord * n));
@@ -749,7 +776,7 @@ int frg_step_model_grad(frg_ctx *c, const float *model, const float *records, in
 int frg_train_step_model(frg_ctx *c, const float *records, int64_t n, int32_t steps, int32_t batch, float lr,
                          uint64_t seed, float *model, float *loss) {
     if (!c || !records || n <= 0 || steps < 0 || batch <= 0 || !(lr > 0.f) || !model) return FRG_EINVAL;
-    HIPCHK(hipSetDevice(c->device));
+    BEGIN(c);
     // normalisation from the data; Glorot-uniform initial weights from a host generator
     std::vector<float> h((size_t)FRG_STEP_MODEL_SIZE, 0.f);
     record_stats(records, n, h.data() + frg::kMlpXmu);

@@ -14,6 +14,9 @@
  * Conventions: functions return FRG_OK (0) or a negative FRG_E* code; per-path outcomes are in
  * the err[] output.  The caller owns every host buffer; a context owns its device memory and
  * stream.  A context is used by one host thread at a time; separate contexts are independent.
+ * Work that an entry point leaves running on a caller's stream (frg_track_device,
+ * frg_sort_dedupe_merge_device) uses the context's buffers: the next call on the same context,
+ * whatever its stream, first waits for that work to finish (frg_last_kernel_ms excepted).
  */
 #ifndef FRG_H
 #define FRG_H
@@ -110,7 +113,8 @@ int frg_track(frg_ctx *ctx, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, ui
 
 /* The same with device pointers (no host copies); output buffers must hold n_paths records.
  * `stream` is a hipStream_t (NULL: the context's stream).  The call returns after the work is
- * enqueued; synchronise on the stream before reading. */
+ * enqueued; synchronise on the stream before reading.  The next call on the context waits for
+ * this launch (it reads the context's instance buffers and work-queue counter). */
 int frg_track_device(frg_ctx *ctx, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
                      int64_t n_paths, frg_c64 *d_fpt, int32_t *d_err, int32_t *d_path_stats, void *stream);
 

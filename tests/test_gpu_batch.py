@@ -66,3 +66,29 @@ def test_autotune_keeps_the_fastest_shape():
         assert np.array_equal(a[0].view(np.float64), b[0].view(np.float64))
     finally:
         t.close()
+
+
+def test_device_launches_on_two_streams():
+    """Back-to-back frg_track_device calls on two caller streams, no synchronisation between
+    them: the second call must not overwrite the instance buffers or the work-queue counter of
+    the first launch while it runs, so both results equal separate synchronous launches."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    (fa, ga), (fb, gb) = tasks.seeded_task(31), tasks.seeded_task(32)
+    n = 4096
+    with capi.Tracker(0, schedule=capi.SCHED_PERSISTENT) as t:
+        ref_a = t.track(fa, ga, seed=1, n=n, want_stats=False)
+        ref_b = t.track(fb, gb, seed=2, n=n, want_stats=False)
+        bufs = [(torch.empty((n, 21), dtype=torch.complex128, device=dev), torch.empty(n, dtype=torch.int32, device=dev))
+                for _ in range(2)]
+        s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+        torch.cuda.synchronize()
+        say(f"[gpu] frg_track_device on two streams, n={n} each")
+        t.track_device(fa, ga, 1, n, bufs[0][0].data_ptr(), bufs[0][1].data_ptr(), stream=s1.cuda_stream)
+        t.track_device(fb, gb, 2, n, bufs[1][0].data_ptr(), bufs[1][1].data_ptr(), stream=s2.cuda_stream)
+        s1.synchronize()
+        s2.synchronize()
+        for (f, e), ref in zip(bufs, (ref_a, ref_b)):
+            assert np.array_equal(e.cpu().numpy(), ref[1])
+            assert np.array_equal(f.cpu().numpy().view(np.float64), ref[0].view(np.float64), equal_nan=True)
