@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/frg_oracle.c) before it is trusted as the GPU checker.
+"""Pins the CPU oracle (oracle/frg_oracle.c) before GPU results are compared against it.
 
 Every check comes from outside the oracle's own formulas: published known-answer vectors,
 numpy/LAPACK, finite differences, invariants of the problem (start points and their cognates
