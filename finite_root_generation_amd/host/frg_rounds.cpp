@@ -15,11 +15,15 @@
 // streams, so an interrupted run continues with fresh paths and every result is reproducible.
 //
 //   frg_rounds NROUNDS [--paths N] [--seed S] [--fpm FILE] [--gamma RE,IM] [--dir DIR]
-//                      [--persistent] [--mixed] [--device D] [--tol T]
+//                      [--persistent] [--mixed] [--learned] [--train-paths M] [--device D] [--tol T]
 //
 // --fpm reads the target as 30 complex doubles (480 bytes, the layout of Vect<nP>); the default is
-// the reference's target (frg.cu:149-178) and gamma (frg.cu:41).
+// the reference's target (frg.cu:149-178) and gamma (frg.cu:41).  --learned trains the step-size
+// model before the first round, on the step logs of M paths (default 32768) of the same target
+// tracked from path ids 2^62 on (never used by a round), and tracks every round with it.
+#include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -62,7 +66,8 @@ struct Args {
     uint64_t seed = 1;
     std::string fpm_file, dir = ".";
     frg_c64 gamma = kRefGamma;
-    bool persistent = false, mixed = false;
+    bool persistent = false, mixed = false, learned = false;
+    int64_t train_paths = 32768;
     int device = 0;
     double tol = 1e-6;  // frgsupport.cu:7
 };
@@ -70,7 +75,7 @@ struct Args {
 [[noreturn]] void usage(const char *msg) {
     std::fprintf(stderr,
                  "%s\nusage: frg_rounds NROUNDS [--paths N] [--seed S] [--fpm FILE] [--gamma RE,IM] [--dir DIR]\n"
-                 "                  [--persistent] [--mixed] [--device D] [--tol T]\n",
+                 "                  [--persistent] [--mixed] [--learned] [--train-paths M] [--device D] [--tol T]\n",
                  msg);
     std::exit(2);
 }
@@ -94,9 +99,12 @@ Args parse(int argc, char **argv) {
             if (std::sscanf(val(), "%lf,%lf", &a.gamma.re, &a.gamma.im) != 2) usage("bad --gamma");
         } else if (k == "--persistent") a.persistent = true;
         else if (k == "--mixed") a.mixed = true;
+        else if (k == "--learned") a.learned = true;
+        else if (k == "--train-paths") a.train_paths = std::atoll(val());
         else usage(("unknown option " + k).c_str());
     }
     if (a.paths <= 0) usage("--paths must be positive");
+    if (a.train_paths <= 0) usage("--train-paths must be positive");
     return a;
 }
 
@@ -184,6 +192,29 @@ int main(int argc, char **argv) {
     if (rc != FRG_OK) {
         std::cerr << "frg_create: " << frg_strerror(rc) << std::endl;
         return 1;
+    }
+
+    if (a.learned) {
+        // step logs of the classical controller on the same target, then the model (GPU kernels)
+        const auto t0 = std::chrono::steady_clock::now();
+        const int64_t cap = std::min<int64_t>(a.train_paths * 128, INT32_MAX);
+        std::vector<float> rec((size_t)cap * FRG_STEP_RECORD);
+        int64_t nrec = 0;
+        rc = frg_collect_step_log(ctx, fpm.data(), a.gamma, a.seed, (uint64_t)1 << 62, a.train_paths, rec.data(), cap,
+                                  &nrec);
+        std::vector<float> model(FRG_STEP_MODEL_SIZE);
+        if (rc == FRG_OK && nrec == 0) rc = FRG_EINVAL;
+        if (rc == FRG_OK) rc = frg_train_step_model(ctx, rec.data(), nrec, 4000, 4096, 2e-3f, 1, model.data(), nullptr);
+        if (rc == FRG_OK) rc = frg_set_step_model(ctx, model.data());
+        o.step_control = FRG_STEP_LEARNED;
+        if (rc == FRG_OK) rc = frg_set_opts(ctx, &o);
+        if (rc != FRG_OK) {
+            std::cerr << "step model: " << frg_strerror(rc) << std::endl;
+            frg_destroy(ctx);
+            return 1;
+        }
+        std::cout << "Step model trained on " << nrec << " steps of " << a.train_paths << " paths in "
+                  << seconds_since(t0) << " sec" << std::endl;
     }
 
     const int64_t n = a.paths;

@@ -1,6 +1,7 @@
 """GPU tests of the round driver (finite_root_generation_amd/frg_rounds): the reference's resume
-files fptColl / errorCodesAgr / progress (frg.cu:295-336, 467-512), a resumed run, and the same
-collection recomputed through the C ABI from the same path ids."""
+files fptColl / errorCodesAgr / progress (frg.cu:295-336, 467-512), a resumed run, the same
+collection recomputed through the C ABI from the same path ids, and a run with the learned
+step-size model trained by the driver itself."""
 import os
 import re
 import subprocess
@@ -10,6 +11,7 @@ import numpy as np
 import pytest
 
 from finite_root_generation_amd import capi, tasks
+from tests import stephenson_np as S
 
 pytestmark = pytest.mark.gpu
 
@@ -79,3 +81,25 @@ def test_rounds_files_and_resume(tracker, tmp_path):
     assert roots == int((codes == 0).sum()) and dups == int((codes == 6).sum()) and trials == 3 * N
     est = float(re.search(r"Roots collected \(est\.\):\s+([0-9.eE+-]+)%", last).group(1))
     assert abs(est - 100 * capi.completeness(roots, dups)) <= 1e-3 * max(1.0, est)
+
+
+def test_rounds_with_learned_step_control(tracker, tmp_path):
+    """--learned: the driver trains the model on its own step logs (path ids 2^62 on), then tracks
+    the round with it.  The collection holds roots of the target, and nearly every root of the
+    plain run of the same path ids."""
+    out = run_rounds(tmp_path, 1, "--persistent", "--learned", "--train-paths", "4096")
+    assert "Step model trained on" in out
+    coll, codes, progress = read_files(tmp_path)
+    assert codes.shape == (N,)
+    assert coll.shape[0] == int((codes == 0).sum())
+    assert np.all(np.diff(coll[:, 0].real) >= 0)
+    F = S.fsys(np.concatenate([coll, np.ones((coll.shape[0], 1))], axis=1), tasks.REF_FPM[None, :])
+    rel = np.max(np.abs(F), axis=1) / (1.0 + np.max(np.abs(coll), axis=1)) ** 3
+    assert rel.max() < 1e-9
+    ecoll, _ = expected(tracker, 1)
+    found = 0
+    for r in ecoll:
+        d = np.max(np.abs(coll - r[None]) / np.maximum(1.0, np.abs(r[None])), axis=1)
+        found += bool(d.min() <= 1e-8)
+    say(f"    learned collection {coll.shape[0]} roots, plain {ecoll.shape[0]}; plain roots found {found}")
+    assert found >= 0.97 * ecoll.shape[0]
