@@ -15,12 +15,17 @@
 // streams, so an interrupted run continues with fresh paths and every result is reproducible.
 //
 //   frg_rounds NROUNDS [--paths N] [--seed S] [--fpm FILE] [--gamma RE,IM] [--dir DIR]
-//                      [--persistent] [--mixed] [--learned] [--train-paths M] [--device D] [--tol T]
+//                      [--persistent] [--mixed] [--learned] [--train-paths M] [--rounds-per-launch R]
+//                      [--device D] [--tol T]
 //
 // --fpm reads the target as 30 complex doubles (480 bytes, the layout of Vect<nP>); the default is
 // the reference's target (frg.cu:149-178) and gamma (frg.cu:41).  --learned trains the step-size
 // model before the first round, on the step logs of M paths (default 32768) of the same target
 // tracked from path ids 2^62 on (never used by a round), and tracks every round with it.
+// --rounds-per-launch R tracks R rounds in one launch (frg_track_batch, one instance per round,
+// same target) and then processes them one by one: the files are the same as with one launch per
+// round, but a launch lasts at least as long as its longest path, so small rounds (the reference's
+// 20,480 paths) leave the GPU mostly idle unless several share a launch.
 #include <algorithm>
 #include <chrono>
 #include <climits>
@@ -68,6 +73,7 @@ struct Args {
     frg_c64 gamma = kRefGamma;
     bool persistent = false, mixed = false, learned = false;
     int64_t train_paths = 32768;
+    int rounds_per_launch = 1;
     int device = 0;
     double tol = 1e-6;  // frgsupport.cu:7
 };
@@ -75,7 +81,8 @@ struct Args {
 [[noreturn]] void usage(const char *msg) {
     std::fprintf(stderr,
                  "%s\nusage: frg_rounds NROUNDS [--paths N] [--seed S] [--fpm FILE] [--gamma RE,IM] [--dir DIR]\n"
-                 "                  [--persistent] [--mixed] [--learned] [--train-paths M] [--device D] [--tol T]\n",
+                 "                  [--persistent] [--mixed] [--learned] [--train-paths M] [--rounds-per-launch R]\n"
+                 "                  [--device D] [--tol T]\n",
                  msg);
     std::exit(2);
 }
@@ -101,10 +108,12 @@ Args parse(int argc, char **argv) {
         else if (k == "--mixed") a.mixed = true;
         else if (k == "--learned") a.learned = true;
         else if (k == "--train-paths") a.train_paths = std::atoll(val());
+        else if (k == "--rounds-per-launch") a.rounds_per_launch = std::atoi(val());
         else usage(("unknown option " + k).c_str());
     }
     if (a.paths <= 0) usage("--paths must be positive");
     if (a.train_paths <= 0) usage("--train-paths must be positive");
+    if (a.rounds_per_launch <= 0) usage("--rounds-per-launch must be positive");
     return a;
 }
 
@@ -218,27 +227,47 @@ int main(int argc, char **argv) {
     }
 
     const int64_t n = a.paths;
-    std::vector<frg_c64> fpt((size_t)n * FRG_NV), merged;
-    std::vector<int32_t> err((size_t)n);
+    const int per_launch = std::min(a.rounds_per_launch, std::max(a.rounds, 1));
+    std::vector<frg_c64> fpt_all((size_t)per_launch * n * FRG_NV), merged;
+    std::vector<int32_t> err_all((size_t)per_launch * n);
+    std::vector<frg_c64> fpms, gammas;
+    for (int i = 0; i < per_launch; ++i) {
+        fpms.insert(fpms.end(), fpm.begin(), fpm.end());
+        gammas.push_back(a.gamma);
+    }
+    std::vector<uint64_t> seeds(per_launch, a.seed), first_ids(per_launch);
+    std::vector<int64_t> counts(per_launch, n);
+    double t_track = 0.0;
     std::cout << std::endl;
     for (int r = 1; r <= a.rounds; ++r) {
         const int64_t k = round + r - 1;  // rounds done before this one
-        std::cout << "Round " << k + 1 << std::endl;
-        frg_stats st;
-        rc = frg_track(ctx, fpm.data(), a.gamma, a.seed, (uint64_t)(k * n), n, fpt.data(), err.data(), nullptr, &st);
-        if (rc != FRG_OK) {
-            std::cerr << "!!! Homotopy GPU failure !!! " << frg_strerror(rc) << std::endl;
-            frg_destroy(ctx);
-            return 1;
+        const int slot = (r - 1) % per_launch;
+        if (slot == 0) {
+            // track this round and the next ones of the launch (round k + i: path ids (k+i)*n ...)
+            const int rr = std::min(per_launch, a.rounds - r + 1);
+            for (int i = 0; i < rr; ++i) first_ids[i] = (uint64_t)((k + i) * n);
+            frg_stats st;
+            rc = rr == 1 ? frg_track(ctx, fpm.data(), a.gamma, a.seed, first_ids[0], n, fpt_all.data(), err_all.data(),
+                                     nullptr, &st)
+                         : frg_track_batch(ctx, rr, fpms.data(), gammas.data(), seeds.data(), first_ids.data(),
+                                           counts.data(), fpt_all.data(), err_all.data(), nullptr, &st);
+            if (rc != FRG_OK) {
+                std::cerr << "!!! Homotopy GPU failure !!! " << frg_strerror(rc) << std::endl;
+                frg_destroy(ctx);
+                return 1;
+            }
+            t_track = st.total_ms * 1e-3 / rr;  // each round's share of the launch
         }
-        const double t_track = st.total_ms * 1e-3;
+        const frg_c64 *fpt = fpt_all.data() + (size_t)slot * n * FRG_NV;
+        int32_t *err = err_all.data() + (size_t)slot * n;
+        std::cout << "Round " << k + 1 << std::endl;
         std::cout << n << " paths tracked in " << t_track << " sec" << std::endl << std::endl;
 
         // sort, delete duplicates, merge (frg.cu:419-440)
         auto t0 = std::chrono::steady_clock::now();
         const int64_t m = (int64_t)(coll.size() / FRG_NV);
         merged.resize((size_t)(m + n) * FRG_NV);
-        const int64_t mm = frg_sort_dedupe_merge(coll.data(), m, fpt.data(), err.data(), n, a.tol, merged.data());
+        const int64_t mm = frg_sort_dedupe_merge(coll.data(), m, fpt, err, n, a.tol, merged.data());
         if (mm < 0) {
             std::cerr << "frg_sort_dedupe_merge failed" << std::endl;
             frg_destroy(ctx);
@@ -247,7 +276,7 @@ int main(int argc, char **argv) {
         merged.resize((size_t)mm * FRG_NV);
         coll.swap(merged);
         Tally rt;
-        rt.add(err.data(), n);
+        rt.add(err, n);
         for (int c = 0; c < 7; ++c) total.code[c] += rt.code[c];
         trials += n;
         const double ratio = (double)total.code[0] / ((double)total.code[0] + (double)total.code[6]);
@@ -262,7 +291,7 @@ int main(int argc, char **argv) {
         }
         {
             std::ofstream f(path_in(a, "errorCodesAgr"), std::ios::binary | std::ios::app);
-            f.write(reinterpret_cast<const char *>(err.data()), (std::streamsize)(n * sizeof(int32_t)));
+            f.write(reinterpret_cast<const char *>(err), (std::streamsize)(n * sizeof(int32_t)));
         }
         const double t_write = seconds_since(t0);
 

@@ -125,6 +125,8 @@ def test_round_driver_is_built_and_fails_loudly_without_gpu(tmp_path):
     assert r.returncode == 2
     r = subprocess.run([exe, "1", "--learned", "--train-paths", "0"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "--train-paths" in r.stderr
+    r = subprocess.run([exe, "1", "--rounds-per-launch", "0"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "--rounds-per-launch" in r.stderr
     if not torch_has_gpu():
         r = subprocess.run([exe, "1", "--paths", "8", "--dir", str(tmp_path)], capture_output=True, text=True,
                            timeout=120)

@@ -103,3 +103,14 @@ def test_rounds_with_learned_step_control(tracker, tmp_path):
         found += bool(d.min() <= 1e-8)
     say(f"    learned collection {coll.shape[0]} roots, plain {ecoll.shape[0]}; plain roots found {found}")
     assert found >= 0.97 * ecoll.shape[0]
+
+
+def test_rounds_per_launch_gives_the_same_files(tracker, tmp_path):
+    """--rounds-per-launch 2 over three rounds (one launch of two rounds, one of one): the files
+    equal those of one launch per round, path id for path id."""
+    run_rounds(tmp_path, 3, "--persistent", "--rounds-per-launch", "2")
+    coll, codes, progress = read_files(tmp_path)
+    ecoll, ecodes = expected(tracker, 3)
+    assert np.array_equal(codes, ecodes)
+    assert np.array_equal(coll.view(np.float64), ecoll.view(np.float64))
+    assert progress.count("-----Progress statistics for round ") == 3
