@@ -551,9 +551,22 @@ __device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, 
 // State-machine transitions (htpytrack.cu:83-216).  Each returns with the next solve set up or
 // with P.done set.
 // ------------------------------------------------------------------------------------------
+// A stage point is Yc + s * sum_m kA[stage][m] k_m over all seven slots, with zero coefficients
+// for the stages not computed yet in this attempt (DormandPrince65 sums only the earlier ones,
+// rungekutta.cu:43-104).  Clearing those slots when an attempt starts keeps a stale non-finite
+// value (the YN of a failed corrector, a stage of a rejected attempt) out of the sum: 0 * NaN is
+// NaN.
+template <int kFirst>
+__device__ __forceinline__ void clear_stages(PathState &P) {
+#pragma unroll
+    for (int m = kFirst; m < 8; ++m)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) P.K[m][k] = mk(0.0, 0.0);
+}
 __device__ __forceinline__ void start_rk(PathState &P) {
     P.phase = PH_RK;
     P.stage = 0;
+    clear_stages<0>(P);
 }
 __device__ __forceinline__ void main_loop_top(const TrackOpts &o, PathState &P) {
     if (P.gs->step == o.max_steps) {  // :85
@@ -634,6 +647,7 @@ __device__ __forceinline__ void finish_rk(int role, const TrackOpts &o, PathStat
         // the rejected attempt's k1, still in K[0]: go straight to the second stage
         P.phase = PH_RK;
         P.stage = 1;
+        clear_stages<1>(P);
         return;
     }
     const double tn = grp_t(role, P.K[1]);

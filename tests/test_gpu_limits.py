@@ -60,3 +60,29 @@ def test_limits_match_oracle(opts, code):
         assert np.mean(d <= 1e-8) >= 0.95
     for f in ("steps", "rk_attempts", "redraws"):
         assert np.mean(st_g[f] == st_o[f]) >= 0.5, f
+
+
+# Paths of the full-size workload (seeded task 1, start seed 2) whose corrector fails with a
+# non-finite point and whose next attempt then starts from the stage slots of the failed or
+# rejected one.  A stage point must combine only the attempt's own stages (0 * NaN is NaN); before
+# the slots were cleared, these paths ran into the ministep limit (code 2).
+STALE_STAGE_PATHS = [5641, 211348, 255766, 270157, 399225, 480135, 529064]
+
+
+def test_stale_stage_values_do_not_leak():
+    fpm, gamma = tasks.seeded_task(1)
+    oo = O.default_opts()
+    spt = np.zeros((len(STALE_STAGE_PATHS), 21), complex)
+    spm = np.zeros((len(STALE_STAGE_PATHS), 30), complex)
+    for k, i in enumerate(STALE_STAGE_PATHS):
+        spt[k], spm[k] = O.generate_start(oo, 2, i)
+    say(f"[gpu] paths after non-finite corrector failures: {STALE_STAGE_PATHS}")
+    err_g, err_o = [], []
+    with capi.Tracker(0) as t:
+        for k, i in enumerate(STALE_STAGE_PATHS):
+            _, e, _, _, _ = t.track_explicit(fpm, gamma, spt[k:k + 1], spm[k:k + 1], seed=2, first_path_id=i)
+            err_g.append(int(e[0]))
+            err_o.append(int(O.homotopy_explicit(oo, fpm, gamma, spt[k:k + 1], spm[k:k + 1], 2, i)[1][0]))
+    say(f"    codes gpu {err_g} oracle {err_o}")
+    assert err_g == err_o
+    assert 2 not in err_g
