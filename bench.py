@@ -162,7 +162,9 @@ def traffic_probe(n):
     import torch
 
     from finite_root_generation_amd import capi, tasks
-    step_model, _ = train_step_model(capi, tasks)
+    log("[probe] training the step model")
+    step_model, info = train_step_model(capi, tasks)
+    log(f"[probe] trained in {info['seconds']} s")
     fpm, gamma = tasks.seeded_task(1)
     dev = torch.device("cuda:0")
     fpt = torch.empty((n, 21), dtype=torch.complex128, device=dev)
