@@ -195,6 +195,26 @@ def test_track_generated_seeded_task(tracker):
     _compare_tracks(fpt_g, err_g, fpt_o, err_o, explain=(oo, fpm, gamma, spt, spm, 7, 0))
 
 
+def test_track_gaussian_start_draws(tracker):
+    """Whole paths with the standard complex Gaussian start draws (FRG_START_GAUSSIAN) instead of
+    the reference's uniform ones: codes and roots against the oracle run in the same mode."""
+    n = 256
+    fpm, gamma = tasks.seeded_task(5)
+    oo = O.default_opts(start_mode=1)
+    say(f"[gpu] track with Gaussian start draws n={n}")
+    tracker.set_opts(start_mode=1)
+    try:
+        fpt_g, err_g, _, _ = tracker.track(fpm, gamma, seed=13, n=n)
+    finally:
+        tracker.set_opts(start_mode=0)
+    fpt_o, err_o, _, _ = O.homotopy(oo, fpm, gamma, 13, 0, n)
+    spt = np.zeros((n, 21), complex)
+    spm = np.zeros((n, 30), complex)
+    for i in range(n):
+        spt[i], spm[i] = O.generate_start(oo, 13, i)
+    _compare_tracks(fpt_g, err_g, fpt_o, err_o, explain=(oo, fpm, gamma, spt, spm, 13, 0))
+
+
 def test_path_statistics_match_oracle(tracker):
     """Per-path counters (accepted steps, RK attempts, Newton calls and iterations, ministeps,
     patch re-draws, tangent solves) against the oracle's TrackPath counts (its IVP calls less one per
