@@ -25,7 +25,8 @@
 // --rounds-per-launch R tracks R rounds in one launch (frg_track_batch, one instance per round,
 // same target) and then processes them one by one: the files are the same as with one launch per
 // round, but a launch lasts at least as long as its longest path, so small rounds (the reference's
-// 20,480 paths) leave the GPU mostly idle unless several share a launch.
+// 20,480 paths) leave the GPU mostly idle unless several share a launch.  The driver then holds R
+// rounds of results in host memory (R x N x 340 bytes).
 #include <algorithm>
 #include <chrono>
 #include <climits>
