@@ -92,3 +92,20 @@ def test_device_launches_on_two_streams():
         for (f, e), ref in zip(bufs, (ref_a, ref_b)):
             assert np.array_equal(e.cpu().numpy(), ref[1])
             assert np.array_equal(f.cpu().numpy().view(np.float64), ref[0].view(np.float64), equal_nan=True)
+
+
+def test_batch_with_empty_and_ragged_instances(tracker):
+    """Instances of 0, 1 and 13 paths between larger ones: every instance equals its own launch,
+    and an empty instance takes no output rows."""
+    counts = [37, 0, 1, 13, 0, 250]
+    inst = _instances(counts, base=60)
+    say(f"[gpu] batch with counts {counts}")
+    with capi.Tracker(0, schedule=capi.SCHED_PERSISTENT) as t:
+        fpt, err, st, summ = t.track_batch(*inst)
+        assert fpt.shape[0] == sum(counts)
+        off = 0
+        for k, c in enumerate(counts):
+            if c:
+                one = t.track(inst[0][k], inst[1][k], seed=inst[2][k], n=c, first_path_id=inst[3][k])
+                assert _same((fpt[off:off + c], err[off:off + c], st[off:off + c]), one), k
+            off += c
