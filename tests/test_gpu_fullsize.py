@@ -78,3 +78,20 @@ def test_outputs_are_canonical_cognates_and_match_the_oracle(full_runs):
         agree += bool(same)
     say(f"    oracle agrees on {agree}/{len(sample)} sampled paths")
     assert agree >= len(sample) - 1
+
+
+def test_device_dedupe_at_full_size(full_runs, tracker):
+    """The 2^20 roots merged into an empty collection and then again into the result (every root
+    a duplicate the second time): device and host rules give the same bits."""
+    from tests.test_gpu_dedupe import _same, device_merge
+    _, _, out = full_runs
+    fpt, err, _ = out["persistent"]
+    coll_h = coll_d = np.zeros((0, 21), complex)
+    for r in range(2):
+        say(f"[gpu] device dedupe at full size, pass {r}")
+        coll_h, err_h = capi.sort_dedupe_merge(coll_h, fpt, err.copy())
+        coll_d, err_d = device_merge(tracker, coll_d, fpt, err.copy())
+        assert np.array_equal(err_h, err_d)
+        assert _same(coll_h, coll_d)
+        say(f"    collection {coll_h.shape[0]} roots, marked duplicate {(err_h == 6).sum()}")
+    assert (err_h == 6).sum() == int((err == 0).sum())
