@@ -35,7 +35,7 @@ EXPORTS = ["frg_default_opts", "frg_create", "frg_destroy", "frg_set_opts", "frg
            "frg_track_explicit", "frg_track_batch", "frg_generate_starts", "frg_solve_step",
            "frg_sort_dedupe_merge", "frg_lambertw", "frg_completeness", "frg_strerror", "frg_version",
            "frg_last_kernel_ms", "frg_flops", "frg_collect_step_log", "frg_train_step_model",
-           "frg_set_step_model", "frg_step_model_grad", "frg_get_opts", "frg_autotune", "frg_sort_dedupe_merge_device"]
+           "frg_set_step_model", "frg_step_model_eval", "frg_step_model_grad", "frg_get_opts", "frg_autotune", "frg_sort_dedupe_merge_device"]
 STEP_MODEL_SIZE, STEP_MODEL_WEIGHTS, STEP_RECORD = 4819, 4801, 9
 
 
@@ -153,6 +153,7 @@ def lib() -> ct.CDLL:
         L.frg_train_step_model.argtypes = [P, P, ct.c_int64, ct.c_int32, ct.c_int32, ct.c_float, ct.c_uint64, P, P]
         L.frg_set_step_model.argtypes = [P, P]
         L.frg_step_model_grad.argtypes = [P, P, P, ct.c_int64, P, P]
+        L.frg_step_model_eval.argtypes = [P, P, ct.c_int64, P]
         _lib = L
     return _lib
 
@@ -309,6 +310,13 @@ class Tracker:
         m = np.ascontiguousarray(model, np.float32)
         assert m.size == STEP_MODEL_SIZE
         _check(lib().frg_set_step_model(self._h, _p(m)), "frg_set_step_model")
+
+    def step_model_eval(self, features):
+        """The tracker's inference (loaded model) on n x 8 features: n predicted log step sizes."""
+        x = np.ascontiguousarray(features, np.float32).reshape(-1, 8)
+        y = np.zeros(x.shape[0], np.float32)
+        _check(lib().frg_step_model_eval(self._h, _p(x), x.shape[0], _p(y)), "frg_step_model_eval")
+        return y
 
     def step_model_grad(self, model, records):
         m = np.ascontiguousarray(model, np.float32)

@@ -737,6 +737,20 @@ int frg_set_step_model(frg_ctx *c, const float *model) {
     return FRG_OK;
 }
 
+int frg_step_model_eval(frg_ctx *c, const float *features, int64_t n, float *out) {
+    if (!c || n < 0 || (n > 0 && (!features || !out)) || !c->have_model) return FRG_EINVAL;
+    if (n == 0) return FRG_OK;
+    BEGIN(c);
+    HIPCHK(c->tsamples.ensure(sizeof(float) * frg::kMlpIn * n));
+    HIPCHK(c->tloss.ensure(sizeof(float) * n));
+    HIPCHK(hipMemcpyAsync(c->tsamples.p, features, sizeof(float) * frg::kMlpIn * n, hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(frg::launch_mlp_eval(c->model, c->tsamples.as<float>(), n, c->tloss.as<float>(), c->stream));
+    HIPCHK(hipMemcpyAsync(out, c->tloss.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FRG_OK;
+}
+
 int frg_step_model_grad(frg_ctx *c, const float *model, const float *records, int64_t n, float *grad, float *loss) {
     if (!c || !model || !records || n <= 0 || !grad) return FRG_EINVAL;
     BEGIN(c);

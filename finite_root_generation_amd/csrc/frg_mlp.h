@@ -3,7 +3,9 @@
 // Parameter block (FP32 master copy, also the C ABI's layout):
 //   W1[64][8], b1[64], W2[64][64], b2[64], w3[64], b3          (4801 weights)
 //   xmu[8], xsd[8], ymu, ysd                                     (input/output normalisation)
-// The tracker reads an FP16 copy of the 4801 weights and accumulates in FP32.
+// The tracker reads an FP16 copy of the 4801 weights and accumulates in FP32; the first hidden
+// layer's activations (tanh, in [-1, 1]) are rounded to FP16 so that the second layer runs as
+// FP16 pair dot products with FP32 accumulation (v_dot2_f32_f16).
 //
 // Features of a step proposal, taken when a step is accepted (all FP32):
 //   0 t                      3 Newton iterations of the corrector   6 log |dx/dt| = -log |V_t|
@@ -38,6 +40,12 @@ struct StepModel {
     float xmu[kMlpIn], xsd[kMlpIn], ymu, ysd;
 };
 
+// acc + a.lo * b.lo + a.hi * b.hi for two FP16 pairs packed in 32-bit words (FP32 accumulation).
+__device__ __forceinline__ float dot2_f16(uint32_t a, uint32_t b, float acc) {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    return __builtin_amdgcn_fdot2(__builtin_bit_cast(h2, a), __builtin_bit_cast(h2, b), acc, false);
+}
+
 __device__ __forceinline__ void unpack8(uint4 q, float out[8]) {
     const uint32_t w[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
@@ -53,7 +61,7 @@ __device__ __forceinline__ void unpack8(uint4 q, float out[8]) {
 // per call.  Must be called by all 64 lanes with a wave-uniform `lanes` mask whose set bits are
 // lanes 8g of the pending groups g.  Group g has written its normalised features to
 // area0[g * stride + 0..7] (floats in LDS); the same 128-float area then holds that group's
-// first hidden layer (at +64) and the 64 output-layer terms (at +0).  `gw` is this lane's group
+// first hidden layer (64 FP16 values at float offset +64) and the 64 output-layer terms (at +0).  `gw` is this lane's group
 // in the wave, `role` its lane in the group.  Returns the predicted log step size in the lanes of
 // each pending group.
 __device__ __forceinline__ float mlp_eval_wave(const StepModel &M, float *area0, int stride, uint64_t lanes, int gw,
@@ -85,27 +93,21 @@ __device__ __forceinline__ float mlp_eval_wave(const StepModel &M, float *area0,
         a = fmaf(a1[5], x1.y, a);
         a = fmaf(a1[6], x1.z, a);
         a = fmaf(a1[7], x1.w, a);
-        area[kMlpH + l] = fast_tanh(a);
+        reinterpret_cast<__half *>(area + kMlpH)[l] = __float2half(fast_tanh(a));
     }
     lds_sync();
     // layer 2 and the output-layer terms w3[l] tanh(.) for every pending group
     for (uint64_t m = lanes; m; m &= m - 1) {
         float *area = area0 + (int)(__builtin_ctzll(m) >> 3) * stride;
-        const float4 *hv = reinterpret_cast<const float4 *>(area + kMlpH);
+        const uint4 *hv = reinterpret_cast<const uint4 *>(area + kMlpH);
         float acc = b2;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            float w[8];
-            unpack8(r2[i], w);
-            const float4 h0 = hv[2 * i], h1 = hv[2 * i + 1];
-            acc = fmaf(w[0], h0.x, acc);
-            acc = fmaf(w[1], h0.y, acc);
-            acc = fmaf(w[2], h0.z, acc);
-            acc = fmaf(w[3], h0.w, acc);
-            acc = fmaf(w[4], h1.x, acc);
-            acc = fmaf(w[5], h1.y, acc);
-            acc = fmaf(w[6], h1.z, acc);
-            acc = fmaf(w[7], h1.w, acc);
+            const uint4 wq = r2[i], hq = hv[i];
+            acc = dot2_f16(wq.x, hq.x, acc);
+            acc = dot2_f16(wq.y, hq.y, acc);
+            acc = dot2_f16(wq.z, hq.z, acc);
+            acc = dot2_f16(wq.w, hq.w, acc);
         }
         area[l] = w3 * fast_tanh(acc);
     }

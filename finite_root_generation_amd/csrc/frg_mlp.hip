@@ -144,6 +144,25 @@ __global__ void mlp_to_half_kernel(const float *w, __half *w16, int n) {
     if (p < n) w16[p] = __float2half(w[p]);
 }
 
+// The tracker's inference on given features: one wave per 8 samples, sample g of the wave in
+// group g, exactly as the tracker evaluates pending proposals (normalisation, mlp_eval_wave).
+__global__ __launch_bounds__(64) void mlp_eval_kernel(StepModel M, const float *x, int64_t n, float *y) {
+    __shared__ float area[8][128];
+    const int role = threadIdx.x & 7, g = threadIdx.x >> 3;
+    const int64_t i = (int64_t)blockIdx.x * 8 + g;
+    const bool valid = i < n;
+    if (valid) area[g][role] = (x[i * kMlpIn + role] - M.xmu[role]) / M.xsd[role];
+    const uint64_t lanes = __ballot(valid && role == 0);
+    const float out = mlp_eval_wave(M, &area[0][0], 128, lanes, g, role);
+    if (valid && role == 0) y[i] = out;
+}
+
+hipError_t launch_mlp_eval(const StepModel &m, const float *x, int64_t n, float *y, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(mlp_eval_kernel, dim3((unsigned)((n + 7) / 8)), dim3(64), 0, s, m, x, n, y);
+    return hipGetLastError();
+}
+
 hipError_t launch_mlp_grad(const MlpGradArgs &a, hipStream_t s) {
     const int blocks = (int)((a.batch + kSB - 1) / kSB);
     if (blocks == 0) return hipSuccess;

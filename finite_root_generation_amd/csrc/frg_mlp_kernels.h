@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "frg_mlp.h"
+
 namespace frg {
 
 struct MlpGradArgs {
@@ -27,5 +29,6 @@ struct MlpAdamArgs {
 hipError_t launch_mlp_grad(const MlpGradArgs &a, hipStream_t s);
 hipError_t launch_mlp_adam(const MlpAdamArgs &a, hipStream_t s);
 hipError_t launch_mlp_to_half(const float *w, __half *w16, hipStream_t s);
+hipError_t launch_mlp_eval(const StepModel &m, const float *x, int64_t n, float *y, hipStream_t s);
 
 }  // namespace frg

@@ -206,6 +206,12 @@ int frg_train_step_model(frg_ctx *ctx, const float *records, int64_t n, int32_t 
 /* Loads a model for FRG_STEP_LEARNED tracking (weights stored as FP16 on the device). */
 int frg_set_step_model(frg_ctx *ctx, const float *model);
 
+/* The tracker's own inference (FP16 weights, FP32 accumulation, the kernel code that proposes
+ * steps) on n feature vectors (n x 8 floats, unnormalised, as in a step record) with the loaded
+ * model; out receives the n predicted log step sizes.  No reference counterpart: a parity entry
+ * point for the inlined inference. */
+int frg_step_model_eval(frg_ctx *ctx, const float *features, int64_t n, float *out);
+
 /* Gradient of the mean loss over records 0..n-1 with respect to the 4801 weights (GPU). */
 int frg_step_model_grad(frg_ctx *ctx, const float *model, const float *records, int64_t n, float *grad, float *loss);
 

@@ -38,6 +38,24 @@ def test_grad_kernel_matches_oracle(tracker, records):
     assert np.linalg.norm(g - go) <= 1e-4 * np.linalg.norm(go)
 
 
+def test_inlined_inference_matches_oracle(tracker, records):
+    """The tracker's own inference code (FP16 weights, FP16 first-layer activations, FP32
+    accumulation, as it proposes steps) against the float64 numpy forward pass, on 4,096 logged
+    feature vectors and on a trained and a random model."""
+    rng = np.random.default_rng(1)
+    trained, _ = tracker.train_step_model(records, steps=300, batch=2048, lr=2e-3, seed=2)
+    feats = records[rng.choice(records.shape[0], 4096, replace=False), :8]
+    for name, m in (("trained", trained), ("random", M.random_model(rng, records).astype(np.float32))):
+        tracker.set_step_model(m)
+        y = tracker.step_model_eval(feats).astype(np.float64)
+        yo = M.forward(m.astype(np.float64), feats)
+        d = np.abs(y - yo) / m[M.YSD]  # in the model's normalised output units
+        say(f"    {name} model: |y - oracle| / ysd median {np.median(d):.2e} max {d.max():.2e}; "
+            f"spread of the oracle's y / ysd {np.std(yo) / m[M.YSD]:.2f}")
+        assert np.all(np.isfinite(y))
+        assert np.median(d) < 1e-3 and d.max() < 5e-3
+
+
 def test_training_reduces_loss(tracker, records):
     say("[gpu] train 800 steps")
     model, loss = tracker.train_step_model(records, steps=800, batch=2048, lr=2e-3, seed=1)
