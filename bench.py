@@ -97,8 +97,12 @@ def roofline(capi, res, model, hbm=None):
     tmin = f64 / (P64_TFLOPS * 1e12) + f32 / (P32_TFLOPS * 1e12)
     achieved = (f64 + f32) / sec / 1e12
     peak = (f64 + f32) / tmin / 1e12 if tmin > 0 else P64_TFLOPS
-    return {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 2), "unit": "TFLOP/s",
+    # compute-bound ("mfma" in the line's vocabulary, against HBM); the arithmetic runs on the FP64
+    # vector ALU, no MFMA instruction is used, and MI355X's FP64 peak (78.6 TFLOP/s) is the same
+    # for the vector and the matrix units
+    return {"bound": "mfma", "achieved": round(achieved, 3), "peak": round(peak, 2), "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": hbm["bytes"] if hbm else None,
+            "bound_detail": "FP64 vector ALU (no MFMA instructions); peak = MI355X FP64 78.6 TFLOP/s",
             "traffic_detail": hbm, "flops_per_launch": {"fp64": f64, "fp32": f32}}
 
 
