@@ -34,7 +34,7 @@ PATH_STATS_DTYPE = np.dtype([(f, np.int32) for f in PATH_STATS_FIELDS])
 EXPORTS = ["frg_default_opts", "frg_create", "frg_destroy", "frg_set_opts", "frg_track", "frg_track_device",
            "frg_track_explicit", "frg_track_batch", "frg_generate_starts", "frg_solve_step",
            "frg_sort_dedupe_merge", "frg_lambertw", "frg_completeness", "frg_strerror", "frg_version",
-           "frg_last_kernel_ms", "frg_flops", "frg_collect_step_log", "frg_train_step_model",
+           "frg_last_kernel_ms", "frg_last_refine_ms", "frg_flops", "frg_collect_step_log", "frg_train_step_model",
            "frg_set_step_model", "frg_step_model_eval", "frg_step_model_grad", "frg_get_opts", "frg_autotune", "frg_sort_dedupe_merge_device"]
 STEP_MODEL_SIZE, STEP_MODEL_WEIGHTS, STEP_RECORD = 4819, 4801, 9
 
@@ -65,6 +65,8 @@ class Opts(ct.Structure):
         ("groups_per_cu", ct.c_int32),
         ("step_control", ct.c_int32),
         ("use_graph", ct.c_int32),
+        ("lanes_per_path", ct.c_int32),
+        ("polish_its", ct.c_int32),
     ]
 
 
@@ -84,6 +86,7 @@ class Stats(ct.Structure):
         ("newton_its", ct.c_int64), ("tangent_solves", ct.c_int64),
         ("kernel_ms", ct.c_double), ("total_ms", ct.c_double),
         ("block_size", ct.c_int32), ("grid_size", ct.c_int32), ("schedule", ct.c_int32), ("precision", ct.c_int32),
+        ("refine_ms", ct.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -142,6 +145,7 @@ def lib() -> ct.CDLL:
         L.frg_strerror.restype = ct.c_char_p
         L.frg_version.restype = ct.c_int
         L.frg_last_kernel_ms.argtypes = [P, ct.POINTER(ct.c_double)]
+        L.frg_last_refine_ms.argtypes = [P, ct.POINTER(ct.c_double)]
         L.frg_get_opts.argtypes = [P, ct.POINTER(Opts)]
         L.frg_sort_dedupe_merge_device.argtypes = [P, P, ct.c_int64, P, P, ct.c_int64, ct.c_double, P,
                                                    ct.POINTER(ct.c_int64), P]
@@ -274,6 +278,11 @@ class Tracker:
     def last_kernel_ms(self) -> float:
         v = ct.c_double()
         _check(lib().frg_last_kernel_ms(self._h, ct.byref(v)), "frg_last_kernel_ms")
+        return v.value
+
+    def last_refine_ms(self) -> float:
+        v = ct.c_double()
+        _check(lib().frg_last_refine_ms(self._h, ct.byref(v)), "frg_last_refine_ms")
         return v.value
 
     def autotune(self, fpm, gamma, seed: int, n_sample: int, first_path_id: int = 0):

@@ -48,7 +48,8 @@ struct frg_ctx {
     int device = 0;
     int num_cu = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // ev0 -> ev_mid: the tracking kernel; ev_mid -> ev1: the refinement kernel
+    hipEvent_t ev0 = nullptr, ev_mid = nullptr, ev1 = nullptr;
     // recorded after work that an entry point leaves running on a caller's stream
     // (frg_track_device, frg_sort_dedupe_merge_device); every entry point waits for it before it
     // touches the context's buffers
@@ -108,6 +109,7 @@ frg::TrackOpts to_track_opts(const frg_opts &o) {
     t.sharp_its = o.sharp_its;
     t.sharp_tol = o.sharp_tol;
     t.sharp_attempts = o.sharp_attempts;
+    t.polish_its = o.polish_its;
     t.sharp_trigger = o.sharp_trigger;
     t.nudge_trigger = o.nudge_trigger;
     t.max_steps = o.max_steps;
@@ -139,6 +141,8 @@ int check_opts(const frg_opts &o) {
     if (o.step_control != FRG_STEP_CLASSIC && o.step_control != FRG_STEP_LEARNED) return FRG_EINVAL;
     if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 256 || o.block_size % 64 != 0)) return FRG_EINVAL;
     if (o.groups_per_cu < 0 || o.max_steps < 0 || o.max_ministeps < 0) return FRG_EINVAL;
+    if (o.polish_its < 0 || o.polish_its > 8) return FRG_EINVAL;
+    if (o.lanes_per_path != 0 && o.lanes_per_path != 8) return FRG_EUNSUPPORTED;
     if (!(o.smin > 0.0) || !(o.rk_tol > 0.0)) return FRG_EINVAL;
     return FRG_OK;
 }
@@ -185,6 +189,7 @@ int run_job(frg_ctx *c, frg::Job &job, hipStream_t stream, frg_stats *summary) {
             hipError_t e = hipSuccess;
             if (kind == frg::LK_PERSISTENT) e = frg::launch_reset_counter(job.queue, stream);
             if (e == hipSuccess) e = frg::launch_track(job, kind, block, grid, stream);
+            if (e == hipSuccess) e = frg::launch_refine(job, stream);
             hipGraph_t g = nullptr;
             const hipError_t e2 = hipStreamEndCapture(stream, &g);
             if (e != hipSuccess || e2 != hipSuccess) {
@@ -196,13 +201,18 @@ int run_job(frg_ctx *c, frg::Job &job, hipStream_t stream, frg_stats *summary) {
             std::memcpy(&c->graph_key, &key, sizeof(key));
             c->graph_valid = true;
         }
+        // a graph replays as one unit: ev_mid marks its end too, so the tracking time of a graph
+        // launch includes the refinement
         HIPCHK(hipEventRecord(c->ev0, stream));
         HIPCHK(hipGraphLaunch(c->graph_exec, stream));
+        HIPCHK(hipEventRecord(c->ev_mid, stream));
         HIPCHK(hipEventRecord(c->ev1, stream));
     } else {
         if (kind == frg::LK_PERSISTENT) HIPCHK(frg::launch_reset_counter(job.queue, stream));
         HIPCHK(hipEventRecord(c->ev0, stream));
         HIPCHK(frg::launch_track(job, kind, block, grid, stream));
+        HIPCHK(hipEventRecord(c->ev_mid, stream));
+        HIPCHK(frg::launch_refine(job, stream));
         HIPCHK(hipEventRecord(c->ev1, stream));
     }
     if (summary) {
@@ -305,9 +315,13 @@ int run_host(frg_ctx *c, frg::Job &job, frg_c64 *fpt, int32_t *err, int32_t *pat
     HIPCHK(hipStreamSynchronize(c->stream));
     if (summary) {
         fill_summary_counts(summary, n, err, st);
-        float ms = 0.f;
-        if (n > 0) HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        float ms = 0.f, ms_ref = 0.f;
+        if (n > 0) {
+            HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev_mid));
+            HIPCHK(hipEventElapsedTime(&ms_ref, c->ev_mid, c->ev1));
+        }
         summary->kernel_ms = ms;
+        summary->refine_ms = ms_ref;
         summary->total_ms = ms_since(t0);
     }
     return FRG_OK;
@@ -361,6 +375,8 @@ int frg_default_opts(frg_opts *o) {
     o->groups_per_cu = 0;
     o->step_control = FRG_STEP_CLASSIC;
     o->use_graph = 0;
+    o->lanes_per_path = 0;
+    o->polish_its = 2;
     return FRG_OK;
 }
 
@@ -379,7 +395,8 @@ int frg_create(frg_ctx **out, int device, const frg_opts *opts) {
     c->device = device;
     c->opts = o;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev_mid) != hipSuccess ||
+        hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_busy, hipEventDisableTiming) != hipSuccess) {
         frg_destroy(c);
         return FRG_EHIP;
@@ -406,6 +423,7 @@ void frg_destroy(frg_ctx *c) {
                       &c->tsamples, &c->tnorm, &c->tloss, &c->tidx, &c->tw16, &c->dedupe_tmp};
     for (DevBuf *b : bufs) b->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev_mid) (void)hipEventDestroy(c->ev_mid);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev_busy) (void)hipEventDestroy(c->ev_busy);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -529,7 +547,7 @@ int frg_autotune(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, u
         }
         HIPCHK(hipStreamSynchronize(c->stream));
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev_mid));
         if (k < table_cap) {
             table[k].schedule = cands[i].schedule;
             table[k].block_size = cands[i].block;
@@ -575,7 +593,17 @@ int frg_last_kernel_ms(frg_ctx *c, double *ms) {
     HIPCHK(hipSetDevice(c->device));  // no BEGIN: reads only the events
     HIPCHK(hipEventSynchronize(c->ev1));
     float v = 0.f;
-    HIPCHK(hipEventElapsedTime(&v, c->ev0, c->ev1));
+    HIPCHK(hipEventElapsedTime(&v, c->ev0, c->ev_mid));
+    *ms = v;
+    return FRG_OK;
+}
+
+int frg_last_refine_ms(frg_ctx *c, double *ms) {
+    if (!c || !ms) return FRG_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipEventSynchronize(c->ev1));
+    float v = 0.f;
+    HIPCHK(hipEventElapsedTime(&v, c->ev_mid, c->ev1));
     *ms = v;
     return FRG_OK;
 }

@@ -1,6 +1,6 @@
 // frg_dedupe.hip -- device-side sort, duplicate marking and merge of a round's roots.
 //
-// Replaces the device sort of the new roots on Re z_0 (frg.cu:419-423, thrust::sort_by_key) and
+// Replaces the device sort of the new roots on Re z_0 (frg.cu:432-435, thrust::sort_by_key) and
 // the host SortingDeletingMerging (frgsupport.cu:10-215), with the same result as the host
 // frg_sort_dedupe_merge (bit-identical collection and codes):
 //   1. the new roots are sorted stably on Re z_0 (NaN last) by a radix sort of an order-preserving
@@ -10,15 +10,20 @@
 //      old root;
 //   3. the merged order is cut into runs wherever two consecutive keys differ by more than tol.  A
 //      duplicate scan from a new root with a finite key stops at the first key farther than tol,
-//      so it never leaves the root's run; one thread per run then applies the sequential rule in
-//      the order of the new roots' indices: a code-0 root becomes code 6 if a neighbour reached
-//      before |d Re z_0| > tol is an old root, or a new root of smaller index still at code 0, and
-//      all 42 reals differ by at most tol.  If a code-0 new root has a NaN key its scan is not
-//      bounded, and one thread runs the whole rule serially instead;
+//      so it never leaves the root's run; the run's new roots are put in index order by a radix
+//      sort of (run, index) keys, and one thread per run then applies the sequential rule in that
+//      order: a code-0 root becomes code 6 if a neighbour reached before |d Re z_0| > tol is an old
+//      root, or a new root of smaller index still at code 0, and all 42 reals differ by at most
+//      tol.  A run of L roots costs O(L) plus its duplicate scans (the non-finite keys of failed
+//      paths, which never break a run, form one long run at the end).  If a code-0 new root has a
+//      NaN key its scan is not bounded, and one thread runs the whole rule serially instead;
 //   4. old roots and the new roots still at code 0 are compacted into the merged collection.
 #include <hip/hip_runtime.h>
 
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "frg_dedupe_kernels.h"
 
@@ -150,26 +155,32 @@ __device__ bool is_duplicate(const double2 *coll, int64_t m, const double2 *fpt,
     return false;
 }
 
-// One thread per run: the run's new roots in increasing index order.
+// Key (run, new-root index) and merged position of every new root; the run of merged position i
+// is rid[i] + brk[i] - 1 (rid: exclusive sum of the run starts brk).
+__global__ void run_order_kernel(const int64_t *mid, int64_t m, const int32_t *brk, const int32_t *rid,
+                                 int64_t total, uint64_t *okey, int64_t *opos) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const int64_t g = mid[i];
+    if (g < m) return;
+    const uint64_t run = (uint64_t)(rid[i] + brk[i] - 1);
+    okey[g - m] = (run << 32) | (uint64_t)(g - m);
+    opos[g - m] = i;
+}
+
+// One thread per run with new roots (the first of its run in the (run, index) order): the run's
+// new roots in increasing index order.
 __global__ void dedupe_runs_kernel(const double2 *coll, int64_t m, const double2 *fpt, int32_t *err,
                                    const int64_t *mid, const double *mkey, const int64_t *start, int64_t nruns,
-                                   int64_t total, double tol) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nruns) return;
-    const int64_t s = start[q], e = q + 1 < nruns ? start[q + 1] : total;
-    int64_t last = -1;
-    for (;;) {
-        int64_t t = INT64_MAX, p = -1;
-        for (int64_t i = s; i < e; ++i) {
-            const int64_t g = mid[i];
-            if (g >= m && g - m > last && g - m < t) {
-                t = g - m;
-                p = i;
-            }
-        }
-        if (p < 0) break;
-        last = t;
-        if (err[t] == 0 && is_duplicate(coll, m, fpt, err, mid, mkey, s, e, t, p, tol)) err[t] = 6;
+                                   int64_t total, const uint64_t *okey, const int64_t *opos, int64_t n, double tol) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint64_t run = okey[s] >> 32;
+    if (s > 0 && (okey[s - 1] >> 32) == run) return;
+    const int64_t lo = start[run], hi = (int64_t)run + 1 < nruns ? start[run + 1] : total;
+    for (int64_t k = s; k < n && (okey[k] >> 32) == run; ++k) {
+        const int64_t t = (int64_t)(okey[k] & 0xffffffffull);
+        if (err[t] == 0 && is_duplicate(coll, m, fpt, err, mid, mkey, lo, hi, t, opos[k], tol)) err[t] = 6;
     }
 }
 
@@ -213,10 +224,10 @@ inline unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 size_t dedupe_temp_bytes(int64_t m, int64_t n) {
     const int64_t total = m + n;
     size_t sort_bytes = 0, scan_bytes = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, (const uint64_t *)nullptr, (uint64_t *)nullptr,
-                                             (const int64_t *)nullptr, (int64_t *)nullptr, n > 0 ? n : 1);
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (const int32_t *)nullptr, (int32_t *)nullptr,
-                                           total > 0 ? total : 1);
+    (void)rocprim::radix_sort_pairs(nullptr, sort_bytes, (const uint64_t *)nullptr, (uint64_t *)nullptr,
+                                    (const int64_t *)nullptr, (int64_t *)nullptr, (size_t)(n > 0 ? n : 1));
+    (void)rocprim::exclusive_scan(nullptr, scan_bytes, (const int32_t *)nullptr, (int32_t *)nullptr, 0,
+                                  (size_t)(total > 0 ? total : 1), rocprim::plus<int32_t>());
     const size_t a = 256;
     auto up = [&](size_t x) { return (x + a - 1) / a * a; };
     return up(sizeof(uint64_t) * n) * 2 + up(sizeof(int64_t) * n) * 3 + up(sizeof(int64_t) * total) * 2 +
@@ -257,7 +268,7 @@ hipError_t sort_dedupe_merge_device(const double2 *coll, int64_t m, const double
     if (n > 0) {
         hipLaunchKernelGGL(keys_kernel, dim3(blocks(n, T)), dim3(T), 0, st, fpt, n, keys, idx);
         size_t sb = cub_bytes;
-        e = hipcub::DeviceRadixSort::SortPairs(cub_tmp, sb, keys, skeys, idx, sidx, n, 0, 64, st);
+        e = rocprim::radix_sort_pairs(cub_tmp, sb, keys, skeys, idx, sidx, (size_t)n, 0, 64, st);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(rank_new_kernel, dim3(blocks(n, T)), dim3(T), 0, st, coll, m, m_f, fpt, sidx, n, mid,
                            mkey);
@@ -281,21 +292,28 @@ hipError_t sort_dedupe_merge_device(const double2 *coll, int64_t m, const double
     } else if (n > 0) {
         hipLaunchKernelGGL(breaks_kernel, dim3(blocks(total, T)), dim3(T), 0, st, mkey, total, tol, brk);
         size_t sb = cub_bytes;
-        e = hipcub::DeviceScan::ExclusiveSum(cub_tmp, sb, brk, rid, total, st);
+        e = rocprim::exclusive_scan(cub_tmp, sb, brk, rid, 0, (size_t)total, rocprim::plus<int32_t>(), st);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(run_starts_kernel, dim3(blocks(total, T)), dim3(T), 0, st, brk, rid, total, start);
+        // the new roots of each run in index order: (run, index) keys sorted, reusing the buffers
+        // of the first sort (keys, idx -> skeys, rank)
+        hipLaunchKernelGGL(run_order_kernel, dim3(blocks(total, T)), dim3(T), 0, st, mid, m, brk, rid, total, keys,
+                           idx);
+        sb = cub_bytes;
+        e = rocprim::radix_sort_pairs(cub_tmp, sb, keys, skeys, idx, rank, (size_t)n, 0, 64, st);
+        if (e != hipSuccess) return e;
         int32_t last_rid = 0, last_brk = 0;
         e = hipMemcpyAsync(&last_rid, rid + (total - 1), sizeof(int32_t), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipMemcpyAsync(&last_brk, brk + (total - 1), sizeof(int32_t), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return e;
         const int64_t nruns = (int64_t)last_rid + last_brk;
-        hipLaunchKernelGGL(dedupe_runs_kernel, dim3(blocks(nruns, 64)), dim3(64), 0, st, coll, m, fpt, err, mid, mkey,
-                           start, nruns, total, tol);
+        hipLaunchKernelGGL(dedupe_runs_kernel, dim3(blocks(n, 64)), dim3(64), 0, st, coll, m, fpt, err, mid, mkey,
+                           start, nruns, total, skeys, rank, n, tol);
     }
     hipLaunchKernelGGL(keep_kernel, dim3(blocks(total, T)), dim3(T), 0, st, mid, m, err, total, keep);
     size_t sb = cub_bytes;
-    e = hipcub::DeviceScan::ExclusiveSum(cub_tmp, sb, keep, rid, total, st);
+    e = rocprim::exclusive_scan(cub_tmp, sb, keep, rid, 0, (size_t)total, rocprim::plus<int32_t>(), st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(scatter_kernel, dim3(blocks(total, 8)), dim3(256), 0, st, coll, m, fpt, mid, keep, rid, total,
                        out);

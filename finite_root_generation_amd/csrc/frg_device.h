@@ -83,6 +83,49 @@ __host__ __device__ __forceinline__ cx<R> cvt(cx<S> a) { return mk(R(a.re), R(a.
 template <typename R>
 __host__ __device__ __forceinline__ cx<R> sel(bool c, cx<R> a, cx<R> b) { return mk(c ? a.re : b.re, c ? a.im : b.im); }
 
+// ------------------------------------------------------------------------------------------
+// Double-double arithmetic (value = hi + lo, |lo| <= ulp(hi)/2): Knuth's TwoSum, Dekker's
+// product through a fused multiply-add, and the accurate sum and product of Hida, Li and Bailey
+// ("Algorithms for quad-double precision floating point arithmetic", 2001).  About 106 bits;
+// used for the residual of the final root refinement only.
+// ------------------------------------------------------------------------------------------
+struct dd {
+    double hi, lo;
+    __host__ __device__ dd() : hi(0.0), lo(0.0) {}
+    __host__ __device__ dd(double a) : hi(a), lo(0.0) {}
+    __host__ __device__ dd(double a, double b) : hi(a), lo(b) {}
+};
+__host__ __device__ __forceinline__ dd two_sum(double a, double b) {
+    const double s = a + b, bb = s - a;
+    return dd(s, (a - (s - bb)) + (b - bb));
+}
+__host__ __device__ __forceinline__ dd quick_two_sum(double a, double b) {  // |a| >= |b|
+    const double s = a + b;
+    return dd(s, b - (s - a));
+}
+__host__ __device__ __forceinline__ dd two_prod(double a, double b) {
+    const double p = a * b;
+    return dd(p, __builtin_fma(a, b, -p));
+}
+__host__ __device__ __forceinline__ dd operator+(dd a, dd b) {
+    dd s = two_sum(a.hi, b.hi);
+    const dd t = two_sum(a.lo, b.lo);
+    s = quick_two_sum(s.hi, s.lo + t.hi);
+    return quick_two_sum(s.hi, s.lo + t.lo);
+}
+__host__ __device__ __forceinline__ dd operator-(dd a) { return dd(-a.hi, -a.lo); }
+__host__ __device__ __forceinline__ dd operator-(dd a, dd b) { return a + (-b); }
+__host__ __device__ __forceinline__ dd operator*(dd a, dd b) {
+    const dd p = two_prod(a.hi, b.hi);
+    return quick_two_sum(p.hi, p.lo + (a.hi * b.lo + a.lo * b.hi));
+}
+// 1/a: a double estimate x, then one Newton step x + x (1 - a x) in double-double
+__host__ __device__ __forceinline__ dd recip(dd a) {
+    const double x = 1.0 / a.hi;
+    const dd e = dd(1.0) - a * dd(x);
+    return dd(x) + dd(x) * e;
+}
+
 // Principal square root by the half-angle formulas of classes.cuh:39-45.
 __host__ __device__ __forceinline__ cx<double> csqrt_ref(cx<double> x) {
     double r = sqrt(x.re * x.re + x.im * x.im);

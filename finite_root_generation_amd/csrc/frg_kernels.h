@@ -16,7 +16,8 @@ struct TrackOpts {
     double newt_tol;
     int newt_attempts, sharp_its;
     double sharp_tol;
-    int sharp_attempts, pad0;
+    int sharp_attempts;
+    int polish_its;     // FP64 Newton iterations in the dehomogenising patch after sharpening
     double sharp_trigger, nudge_trigger;
     int max_steps, max_ministeps;
     double patch_lo, patch_hi, start_lo, start_hi;
@@ -72,6 +73,8 @@ hipError_t launch_debug_start(const DebugStartArgs &a, hipStream_t stream);
 // host-side launchers (frg_tracker.hip)
 enum LaunchKind : int { LK_STATIC = 0, LK_PERSISTENT = 1 };
 hipError_t launch_track(const Job &job, int kind, int block, int grid, hipStream_t stream);
+// refinement of code-0 roots and cognate canonicalisation of every output row (after the tracker)
+hipError_t launch_refine(const Job &job, hipStream_t stream);
 // zeroes a work-queue counter with a kernel (capturable, unlike a memset node)
 hipError_t launch_reset_counter(unsigned long long *p, hipStream_t stream);
 size_t track_lds_bytes(int block, int precision);

@@ -57,16 +57,16 @@ __host__ __device__ __forceinline__ cx<R> params_at(const PoseEval<R> &e, cx<R> 
     return e.gamma * (id * id);
 }
 
-// F values only (the residual for Newton, used in FP64 when the Jacobian is FP32).
+// F values of pose j at given parameters q = (P0, P_j, Pc0, Pc_j, Q_j, Qc_j), linkage L[14],
+// homogeniser Z0 and rotation U.  Used over double (the Newton residual in FP64 when the
+// Jacobian is FP32) and over double-double (the residual of the final root refinement).
 template <typename R>
-__host__ __device__ __forceinline__ void pose_values(const PoseEval<R> &e, cx<R> &fI, cx<R> &fII, cx<R> &fIII) {
-    cx<R> q[6];
-    params_at(e, q);
+__host__ __device__ __forceinline__ void pose_residuals(const cx<R> q[6], const cx<R> L[14], cx<R> Z0, cx<R> U,
+                                                        cx<R> &fI, cx<R> &fII, cx<R> &fIII) {
     const cx<R> P0 = q[0], Pj = q[1], Pc0 = q[2], Pcj = q[3], Q = q[4], Qc = q[5];
-    const cx<R> Z0 = e.Z0, U = e.U;
     const cx<R> zu = Z0 * inv(U);
-    const cx<R> A = e.L[0], B = e.L[1], C = e.L[2], D = e.L[3], F = e.L[4], G = e.L[5], H = e.L[6];
-    const cx<R> Ac = e.L[7], Bc = e.L[8], Cc = e.L[9], Dc = e.L[10], Fc = e.L[11], Gc = e.L[12], Hc = e.L[13];
+    const cx<R> A = L[0], B = L[1], C = L[2], D = L[3], F = L[4], G = L[5], H = L[6];
+    const cx<R> Ac = L[7], Bc = L[8], Cc = L[9], Dc = L[10], Fc = L[11], Gc = L[12], Hc = L[13];
     const cx<R> CA = C - A, CcAc = Cc - Ac;
     {
         cx<R> X = Pj * Z0 - A - CA * Q, Y = P0 * Z0 - H;
@@ -92,6 +92,14 @@ __host__ __device__ __forceinline__ void pose_values(const PoseEval<R> &e, cx<R>
         cx<R> W = Z0 * X - U * Y, Wc = Xc - zu * Yc;
         fIII = Z0 * K - W * Wc;
     }
+}
+
+// F values only, at the evaluation point's t.
+template <typename R>
+__host__ __device__ __forceinline__ void pose_values(const PoseEval<R> &e, cx<R> &fI, cx<R> &fII, cx<R> &fIII) {
+    cx<R> q[6];
+    params_at(e, q);
+    pose_residuals(q, e.L, e.Z0, e.U, fI, fII, fIII);
 }
 
 // Quantities shared by the three equations of one pose.

@@ -92,12 +92,14 @@ __device__ __forceinline__ void scale_row(const Row<R> &r, bool newton, cx<doubl
 // tangent one (zeros; w rhs = 1).  fnewt (FP64 residual values of this lane's three
 // equations) is used when `newton`.  Output: this lane's slots of x.
 // ------------------------------------------------------------------------------------------
-template <typename R>
+// kExtRes: the Newton residual comes from fnewt (always for FP32 rows, whose F is not accurate
+// enough; for FP64 rows when the caller evaluates it more accurately) instead of with the rows.
+template <typename R, bool kExtRes = (sizeof(R) != sizeof(double))>
 __device__ __forceinline__ void group_solve(int role, GroupLds<R> *L, const cx<double> Ye[3], const cx<double> uu[3],
                                             const cx<double> ww[3], bool newton, double w_rhs,
                                             const cx<double> *par, cx<double> gamma,
                                             const cx<double> fnewt[3], cx<double> x[3]) {
-    constexpr bool kOwnF = sizeof(R) == sizeof(double);  // FP64 rows carry F themselves
+    constexpr bool kOwnF = !kExtRes;
     const cx<R> zero = mk(R(0), R(0));
     const int pr = role < 7 ? role : 6;  // LDS row index used by this lane for pose data
 
@@ -468,22 +470,25 @@ __device__ __forceinline__ void generate_start(int role, const TrackOpts &o, Pat
     }
 }
 
+// Task instance of a global path index: the last instance whose first path is <= path.
+__device__ __forceinline__ int instance_of(const Job &J, int64_t path) {
+    if (J.n_inst <= 1) return 0;
+    int lo = 0, hi = J.n_inst - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (J.inst_first[mid] <= path) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // ------------------------------------------------------------------------------------------
 // Path set-up (Homotopy kernel start + TrackPath initialisation, htpytrack.cu:45-77)
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, int64_t path) {
     const TrackOpts &o = J.o;
     P.gs->path = path;
-    int inst = 0;
-    if (J.n_inst > 1) {
-        int lo = 0, hi = J.n_inst - 1;  // last instance whose first path <= path
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (J.inst_first[mid] <= path) lo = mid;
-            else hi = mid - 1;
-        }
-        inst = lo;
-    }
+    const int inst = instance_of(J, path);
     P.gs->inst = inst;
     const int64_t local = path - (J.n_inst > 1 ? J.inst_first[inst] : 0);
     P.gs->seed = J.seed[inst];
@@ -785,8 +790,23 @@ __device__ __forceinline__ void finish_newton(int role, const Job &J, PathState 
     P.gs->streak = 0;
     start_rk(P);
 }
+// The homogeneous end point (before dehomogenisation) for callers that asked for it.
+__device__ __forceinline__ void store_yfinal(int role, const Job &J, int64_t path, const cx<double> Y[3]) {
+    if (!J.yfinal) return;
+    cx<double> *yo = J.yfinal + path * 23;
+    if (role < 7) {
+        yo[2 * role] = Y[0];
+        yo[2 * role + 1] = Y[1];
+        yo[14 + role] = Y[2];
+    } else {
+        yo[21] = Y[0];
+        yo[22] = Y[1];
+    }
+}
+
 // end of a sharpening Newton call (:198-216)
-__device__ __forceinline__ void finish_sharp(int role, const TrackOpts &o, PathState &P) {
+__device__ __forceinline__ void finish_sharp(int role, const Job &J, PathState &P) {
+    const TrackOpts &o = J.o;
     if (P.gs->ministep++ == o.max_ministeps) {
         P.gs->code = 2;
         P.done = true;
@@ -808,89 +828,24 @@ __device__ __forceinline__ void finish_sharp(int role, const TrackOpts &o, PathS
 }
 
 // ------------------------------------------------------------------------------------------
-// Output: dehomogenise (frg.cu:60), CognateSolns (frgkinematics.cu:227-285), store (frg.cu:68).
+// Output of the tracker: dehomogenise (frg.cu:60) and store the 21 values with the code and the
+// counters.  The refinement kernel below then polishes code-0 roots and applies CognateSolns.
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ void cognate12(const cx<double> o[7], cx<double> P, cx<double> c[7]) {
-    const cx<double> A = o[0], B = o[1], C = o[2], D = o[3], F = o[4], G = o[5], H = o[6];
-    cx<double> sc = cdiv(B - F, D - F);
-    const cx<double> Dp = sc * (D - H) + H, Gp = sc * (G - H) + H;
-    sc = cdiv(P - F, H - F);
-    c[0] = sc * (A - B) + B;
-    c[1] = B;
-    c[2] = sc * (C - B) + B;
-    c[3] = sc * (Dp - B) + B;
-    c[4] = B + (P - F);
-    c[5] = sc * (Gp - B) + B;
-    c[6] = sc * (H - B) + B;
-}
-
-template <typename R>
-__device__ __forceinline__ void finish_path(int role, const Job &J, PathState &P, GroupLds<R> *L) {
+__device__ __forceinline__ void finish_path(int role, const Job &J, PathState &P) {
     // output point: Yc for codes 1-4 and code 2 before sharpening, the sharpened point otherwise
     const bool from_sharp = (P.phase == PH_SHARP);
     cx<double> Y[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) Y[k] = sel(from_sharp, P.K[0][k], P.Yc[k]);
     const int64_t path = P.gs->path;
-    if (J.yfinal) {
-        cx<double> *yo = J.yfinal + path * 23;
-        if (role < 7) {
-            yo[2 * role] = Y[0];
-            yo[2 * role + 1] = Y[1];
-            yo[14 + role] = Y[2];
-        } else {
-            yo[21] = Y[0];
-            yo[22] = Y[1];
-        }
-    }
+    store_yfinal(role, J, path, Y);
     // dehomogenise: divide z and Z0 by Z0
-    const cx<double> Z0 = grp_bcast(Y[0], role, 7);
-    const cx<double> iZ0 = inv(Z0);
-    cx<double> zd[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) zd[k] = Y[k] * iZ0;
-    // gather the 14 linkage values through LDS (as doubles, two LDS cells each)
-    cx<R> *cell = L->a;
-    double *dcell = reinterpret_cast<double *>(&L->phat[0][0]);
-    lds_sync();
-    if (role < 7) {
-        dcell[4 * role + 0] = zd[0].re;
-        dcell[4 * role + 1] = zd[0].im;
-        dcell[4 * role + 2] = zd[1].re;
-        dcell[4 * role + 3] = zd[1].im;
-    }
-    lds_sync();
-    cx<double> lk[14];
-#pragma unroll
-    for (int i = 0; i < 14; ++i) lk[i] = mk(dcell[2 * i], dcell[2 * i + 1]);
-    lds_sync();
-    (void)cell;
-    const cx<double> P0 = J.fpm[(int64_t)P.gs->inst * 30 + 0], Pc0 = J.fpm[(int64_t)P.gs->inst * 30 + 8];
-    const cx<double> Pj = J.fpm[(int64_t)P.gs->inst * 30 + (role < 7 ? role + 1 : 7)];
-    cx<double> c1[7], c2[7];
-    cognate12(&lk[0], P0, c1);
-    cognate12(&lk[7], Pc0, c2);
-    const cx<double> B = lk[1], F = lk[4];
-    const cx<double> Udown = cdiv((Pj - B) - zd[2] * (P0 - F), F - B);
-    const bool swap = c1[0].re < lk[0].re;
+    const cx<double> iZ0 = inv(grp_bcast(Y[0], role, 7));
     if (role < 7) {
         cx<double> *out = J.fpt + path * 21;
-        cx<double> cog[14];
-#pragma unroll
-        for (int i = 0; i < 7; ++i) {
-            cog[i] = c1[i];
-            cog[7 + i] = c2[i];
-        }
-        // this lane's two linkage entries
-        cx<double> o0 = zd[0], o1 = zd[1];
-#pragma unroll
-        for (int i = 0; i < 14; ++i) {
-            o0 = sel(swap && i == 2 * role, cog[i], o0);
-            o1 = sel(swap && i == 2 * role + 1, cog[i], o1);
-        }
-        out[2 * role] = o0;
-        out[2 * role + 1] = o1;
-        out[14 + role] = sel(swap, Udown, zd[2]);
+        out[2 * role] = Y[0] * iZ0;
+        out[2 * role + 1] = Y[1] * iZ0;
+        out[14 + role] = Y[2] * iZ0;
     }
     if (role == 0) {
         J.err[path] = P.gs->code;
@@ -998,7 +953,7 @@ __device__ __forceinline__ void micro_step(int role, const Job &J, PathState &P,
     const int max_its = P.phase == PH_SHARP ? o.sharp_its : o.newt_its;
     const double tol = P.phase == PH_SHARP ? o.sharp_tol : o.newt_tol;
     if (P.its < max_its && P.residual > tol) return;  // another iteration
-    if (P.phase == PH_SHARP) finish_sharp(role, o, P);
+    if (P.phase == PH_SHARP) finish_sharp(role, J, P);
     else finish_newton(role, J, P, reinterpret_cast<float *>(reinterpret_cast<GroupLds<double> *>(L)->a));
 }
 
@@ -1069,7 +1024,7 @@ __global__ __launch_bounds__(256) void track_kernel(Job J) {
             }
         }
         if (live && P.done) {
-            finish_path(role, J, P, L);
+            finish_path(role, J, P);
             if (kPersistent) {
                 path = claim_path(role, J.queue);
                 live = path < J.n_total;
@@ -1084,6 +1039,152 @@ __global__ __launch_bounds__(256) void track_kernel(Job J) {
             break;
         }
     }
+}
+
+// ------------------------------------------------------------------------------------------
+// Refinement and output, one 8-lane group per path, launched after the tracker:
+//  * a code-0 root gets polish_its Newton steps at t = 1 in the dehomogenising patch e_21
+//    (frg.cu:44-46) with Z0 = 1: iterative refinement with an extended residual.  The Jacobian
+//    rows, the row scaling and the structured solve are the tracker's FP64 Newton step (start
+//    values = target, so q = fpm exactly and dF/dt = 0; w = e_t with dt = 0); the residual F is
+//    evaluated in double-double at the target and rounded, so the iteration converges to the
+//    double nearest the root even when the root is ill-conditioned.  A step is taken only while it
+//    is no longer than kPolishCap * max(1, |Y|) (the point is in Newton's quadratic basin);
+//  * every path: CognateSolns (frgkinematics.cu:227-285) and the store (frg.cu:63-68).
+// ------------------------------------------------------------------------------------------
+constexpr double kPolishCap = 1e-6;
+
+// Cognate12 (frgkinematics.cu:185-224): the Stephenson II 1/2 cognate of linkage o[0..6] for
+// the coupler point P.
+__device__ __forceinline__ void cognate12(const cx<double> o[7], cx<double> P, cx<double> c[7]) {
+    const cx<double> A = o[0], B = o[1], C = o[2], D = o[3], F = o[4], G = o[5], H = o[6];
+    cx<double> sc = cdiv(B - F, D - F);
+    const cx<double> Dp = sc * (D - H) + H, Gp = sc * (G - H) + H;
+    sc = cdiv(P - F, H - F);
+    c[0] = sc * (A - B) + B;
+    c[1] = B;
+    c[2] = sc * (C - B) + B;
+    c[3] = sc * (Dp - B) + B;
+    c[4] = B + (P - F);
+    c[5] = sc * (Gp - B) + B;
+    c[6] = sc * (H - B) + B;
+}
+
+// The 14 linkage values of the group's point, from this lane's slots (lane r < 7 holds z[2r],
+// z[2r+1]), gathered through the group's LDS area as doubles.
+__device__ __forceinline__ void gather_linkage(int role, GroupLds<double> *L, const cx<double> Y[3],
+                                               cx<double> lk[14]) {
+    double *dcell = reinterpret_cast<double *>(&L->phat[0][0]);
+    lds_sync();
+    if (role < 7) {
+        dcell[4 * role + 0] = Y[0].re;
+        dcell[4 * role + 1] = Y[0].im;
+        dcell[4 * role + 2] = Y[1].re;
+        dcell[4 * role + 3] = Y[1].im;
+    }
+    lds_sync();
+#pragma unroll
+    for (int i = 0; i < 14; ++i) lk[i] = mk(dcell[2 * i], dcell[2 * i + 1]);
+    lds_sync();
+}
+
+__global__ __launch_bounds__(64) void refine_kernel(Job J) {
+    extern __shared__ unsigned char smem[];
+    const int role = threadIdx.x & 7;
+    const int grp = threadIdx.x >> 3;
+    GroupSlot *S = reinterpret_cast<GroupSlot *>(smem) + grp;
+    GroupLds<double> *L = &S->lds;
+    const int64_t path = (int64_t)blockIdx.x * (blockDim.x >> 3) + grp;
+    if (path >= J.n_total) return;  // whole groups
+    const int inst = instance_of(J, path);
+    const cx<double> *f = J.fpm + (int64_t)inst * 30;
+    cx<double> *row = J.fpt + path * 21;
+    const int j = role < 7 ? role + 1 : 7;
+    // this lane's slots of the dehomogenised point; lane 7 holds Z0 = 1 and t = 1
+    cx<double> Y[3];
+    if (role < 7) {
+        Y[0] = row[2 * role];
+        Y[1] = row[2 * role + 1];
+        Y[2] = row[14 + role];
+    } else {
+        Y[0] = mk(1.0, 0.0);
+        Y[1] = mk(1.0, 0.0);
+        Y[2] = mk(0.0, 0.0);
+    }
+    if (J.o.polish_its > 0 && J.err[path] == 0) {
+        const int pidx[6] = {0, j, 8, 8 + j, 15 + j, 22 + j};  // P0, P_j, Pc0, Pc_j, Q_j, Qc_j
+        cx<double> *par = S->par[role];
+        cx<dd> qd[6];
+#pragma unroll
+        for (int p = 0; p < 6; ++p) {
+            par[p] = f[pidx[p]];
+            par[6 + p] = mk(0.0, 0.0);
+            qd[p] = mk(dd(par[p].re), dd(par[p].im));
+        }
+        cx<double> uu[3], ww[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            uu[k] = mk((role == 7 && k == 0) ? 1.0 : 0.0, 0.0);
+            ww[k] = mk((role == 7 && k == 1) ? 1.0 : 0.0, 0.0);
+        }
+        const cx<double> gamma = J.gamma[inst];
+        int taken = 0;
+        for (int it = 0; it < J.o.polish_its; ++it) {
+            cx<double> lk[14];
+            gather_linkage(role, L, Y, lk);
+            cx<dd> Ld[14];
+#pragma unroll
+            for (int i = 0; i < 14; ++i) Ld[i] = mk(dd(lk[i].re), dd(lk[i].im));
+            const cx<dd> Ud = role < 7 ? mk(dd(Y[2].re), dd(Y[2].im)) : mk(dd(1.0), dd(0.0));
+            cx<dd> fI, fII, fIII;
+            pose_residuals<dd>(qd, Ld, mk(dd(1.0), dd(0.0)), Ud, fI, fII, fIII);
+            // rounded to double: hi is the nearest double to hi + lo
+            const cx<double> fN[3] = {mk(fI.re.hi, fI.im.hi), mk(fII.re.hi, fII.im.hi), mk(fIII.re.hi, fIII.im.hi)};
+            cx<double> x[3];
+            group_solve<double, true>(role, L, Y, uu, ww, true, 0.0, par, gamma, fN, x);
+            const double r = grp_norm(x);
+            if (!(r <= kPolishCap * fmax(1.0, grp_norm(Y)))) break;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) Y[k] = Y[k] + x[k];
+            ++taken;
+        }
+        if (role == 0 && J.stats && taken) J.stats[path * 8 + 3] += taken;
+    }
+    // CognateSolns: the root or its 1/2 cognate, whichever has the smaller Re z0
+    cx<double> lk[14];
+    gather_linkage(role, L, Y, lk);
+    const cx<double> P0 = f[0], Pc0 = f[8], Pj = f[j];
+    cx<double> c1[7], c2[7];
+    cognate12(&lk[0], P0, c1);
+    cognate12(&lk[7], Pc0, c2);
+    const cx<double> B = lk[1], F = lk[4];
+    const cx<double> Udown = cdiv((Pj - B) - Y[2] * (P0 - F), F - B);
+    const bool swap = c1[0].re < lk[0].re;
+    if (role < 7) {
+        cx<double> cog[14];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+            cog[i] = c1[i];
+            cog[7 + i] = c2[i];
+        }
+        cx<double> o0 = Y[0], o1 = Y[1];
+#pragma unroll
+        for (int i = 0; i < 14; ++i) {
+            o0 = sel(swap && i == 2 * role, cog[i], o0);
+            o1 = sel(swap && i == 2 * role + 1, cog[i], o1);
+        }
+        row[2 * role] = o0;
+        row[2 * role + 1] = o1;
+        row[14 + role] = sel(swap, Udown, Y[2]);
+    }
+}
+
+hipError_t launch_refine(const Job &job, hipStream_t stream) {
+    const int64_t g = (job.n_total + 7) / 8;
+    if (g == 0) return hipSuccess;
+    if (g > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(refine_kernel, dim3((unsigned)g), dim3(64), 8 * sizeof(GroupSlot), stream, job);
+    return hipGetLastError();
 }
 
 size_t track_lds_bytes(int block, int precision) {

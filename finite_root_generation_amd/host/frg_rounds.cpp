@@ -5,14 +5,17 @@
 //     the aggregate of error codes `errorCodesAgr` (int32 per trial) from the working directory if
 //     they exist, recounts the totals and the number of rounds done (frg.cu:295-336);
 //   * each round: tracks n paths (frg.cu:356-404), sorts, marks duplicates with code 6 and merges
-//     the new roots into the collection (frg.cu:432-440, frgsupport.cu:10-215), updates the totals
-//     and the Lambert-W completeness estimate (frg.cu:442-465), rewrites `fptColl`, appends the
-//     round's codes to `errorCodesAgr` and a statistics block to `progress` (frg.cu:467-512).
+//     the new roots into the collection (frg.cu:432-441, frgsupport.cu:10-215), updates the totals
+//     and the Lambert-W completeness estimate (frg.cu:443-464), rewrites `fptColl` and appends the
+//     round's codes to `errorCodesAgr` (frg.cu:472-480) and a statistics block to `progress`
+//     (frg.cu:486-512).
 // The three files keep the reference's byte and text formats, so a collection started by either
 // program can be continued by the other.
 //
-// Path ids: round k (counting the resumed ones) tracks ids k*n .. k*n + n - 1 of the seed's Philox
-// streams, so an interrupted run continues with fresh paths and every result is reproducible.
+// Path ids: trials are numbered consecutively across runs.  A round that starts after T trials
+// (those recorded in `errorCodesAgr` included) tracks ids T .. T + n - 1 of the seed's Philox
+// streams, so a resumed run continues with fresh paths whatever --paths the earlier runs used, and
+// every result is reproducible.
 //
 //   frg_rounds NROUNDS [--paths N] [--seed S] [--fpm FILE] [--gamma RE,IM] [--dir DIR]
 //                      [--persistent] [--mixed] [--learned] [--train-paths M] [--rounds-per-launch R]
@@ -244,9 +247,10 @@ int main(int argc, char **argv) {
         const int64_t k = round + r - 1;  // rounds done before this one
         const int slot = (r - 1) % per_launch;
         if (slot == 0) {
-            // track this round and the next ones of the launch (round k + i: path ids (k+i)*n ...)
+            // track this round and the next ones of the launch: the i-th starts after trials + i*n
+            // trials, so its path ids follow every id tracked before
             const int rr = std::min(per_launch, a.rounds - r + 1);
-            for (int i = 0; i < rr; ++i) first_ids[i] = (uint64_t)((k + i) * n);
+            for (int i = 0; i < rr; ++i) first_ids[i] = (uint64_t)(trials + i * n);
             frg_stats st;
             rc = rr == 1 ? frg_track(ctx, fpm.data(), a.gamma, a.seed, first_ids[0], n, fpt_all.data(), err_all.data(),
                                      nullptr, &st)
@@ -264,7 +268,7 @@ int main(int argc, char **argv) {
         std::cout << "Round " << k + 1 << std::endl;
         std::cout << n << " paths tracked in " << t_track << " sec" << std::endl << std::endl;
 
-        // sort, delete duplicates, merge (frg.cu:419-440)
+        // sort, delete duplicates, merge (frg.cu:432-441)
         auto t0 = std::chrono::steady_clock::now();
         const int64_t m = (int64_t)(coll.size() / FRG_NV);
         merged.resize((size_t)(m + n) * FRG_NV);
@@ -284,7 +288,7 @@ int main(int argc, char **argv) {
         const double est = frg_completeness(total.code[0], total.code[6]);
         const double t_dedupe = seconds_since(t0);
 
-        // files (frg.cu:467-476)
+        // files (frg.cu:472-480)
         t0 = std::chrono::steady_clock::now();
         {
             std::ofstream f(path_in(a, "fptColl"), std::ios::binary | std::ios::trunc);
@@ -296,7 +300,7 @@ int main(int argc, char **argv) {
         }
         const double t_write = seconds_since(t0);
 
-        // progress (frg.cu:478-508)
+        // progress (frg.cu:486-512)
         std::ofstream p(path_in(a, "progress"), std::ios::app);
         p << "-----Progress statistics for round " << k + 1 << "-----" << std::endl;
         p << std::endl;

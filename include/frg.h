@@ -51,7 +51,7 @@ typedef struct {
 } frg_c64;
 
 /* Tracker settings.  frg_default_opts() fills the reference's #defines (htpytrack.cu:9-31,
- * frgkinematics.cu:7-8) and the FP64 / static / classic choices. */
+ * frgkinematics.cu:7-8), the FP64 / static / classic choices and a 2-iteration final polish. */
 typedef struct {
     double sinit, smax, smin, rk_tol;  /* htpytrack.cu:9-12 */
     int32_t caution_steps, newt_its;   /* :13, :16 */
@@ -71,6 +71,9 @@ typedef struct {
     int32_t groups_per_cu;                 /* persistent grid: paths in flight per CU; 0 = occupancy */
     int32_t step_control;                  /* FRG_STEP_* */
     int32_t use_graph;                     /* 1: launch through a cached hipGraph (queue reset + tracker) */
+    int32_t lanes_per_path;                /* lanes of a wave64 that track one path: 8 or 16; 0 = default (8) */
+    int32_t polish_its;                    /* FP64 Newton iterations on a sharpened root in the dehomogenising
+                                              patch e_21 (frg.cu:44-46) at t = 1; 0 = none, as the reference */
 } frg_opts;
 
 /* Per-call summary. */
@@ -81,6 +84,7 @@ typedef struct {
     double kernel_ms;       /* device time of the tracking launch (HIP events) */
     double total_ms;        /* host wall time of the whole call, copies included */
     int32_t block_size, grid_size, schedule, precision;
+    double refine_ms;       /* device time of the refinement + cognate launch that follows it */
 } frg_stats;
 
 /* Opaque context: device, stream, device buffers, tuning cache. */
@@ -173,7 +177,7 @@ int64_t frg_sort_dedupe_merge(const frg_c64 *coll, int64_t m, const frg_c64 *fpt
                               double tol, frg_c64 *coll_out);
 /* frg_sort_dedupe_merge on the device, with the same rule and a bit-identical result: all four
  * arrays are device pointers (d_coll_out: room for m + n roots), m + n < 2^31, *m_out receives
- * the merged length.  Replaces the device sort of frg.cu:419-423 (thrust::sort_by_key) and the
+ * the merged length.  Replaces the device sort of frg.cu:432-435 (thrust::sort_by_key) and the
  * host SortingDeletingMerging (frgsupport.cu:10-215).  `stream` is a hipStream_t (NULL: the
  * context's stream); the call synchronises it before returning. */
 int frg_sort_dedupe_merge_device(frg_ctx *ctx, const frg_c64 *d_coll, int64_t m, const frg_c64 *d_fpt,
@@ -216,8 +220,12 @@ int frg_step_model_eval(frg_ctx *ctx, const float *features, int64_t n, float *o
 int frg_step_model_grad(frg_ctx *ctx, const float *model, const float *records, int64_t n, float *grad, float *loss);
 
 /* Device time (ms, HIP events) of the last tracking launch of the context; waits for it to end.
- * No reference counterpart (the reference times RNG init + kernel + copies, frg.cu:356-426). */
+ * A launch is the tracking kernel followed by the refinement kernel (FP64 polish of code-0 roots
+ * with an extended residual, then CognateSolns, frgkinematics.cu:227-285); frg_last_kernel_ms
+ * times the first (for a graph launch: both), frg_last_refine_ms the second.  No reference
+ * counterpart (the reference times RNG init + kernel + copies, frg.cu:356-426). */
 int frg_last_kernel_ms(frg_ctx *ctx, double *ms);
+int frg_last_refine_ms(frg_ctx *ctx, double *ms);
 
 /* Algorithmic floating-point operations per unit of work of this implementation's formulation
  * (real +, -, *, / count 1; counted once per path, not per lane).  A tracked path costs

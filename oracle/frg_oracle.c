@@ -135,6 +135,7 @@ void oracle_default_opts(otrk_opts *o) {
     o->start_lo = -5.0;          /* frgkinematics.cu:8 */
     o->start_hi = 5.0;           /* frgkinematics.cu:7 */
     o->start_mode = 0;
+    o->polish_its = 2;
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -149,67 +150,81 @@ enum { iA = 0, iB, iC, iD, iF, iG, iH, iAc, iBc, iCc, iDc, iFc, iGc, iHc };
  *   rows 7..13  II_j  (:57-66)
  *   rows 14..20 III_j (:68-122)
  * The terms are summed in the reference's order. */
-void oracle_fsys(const ocplx *Z, const ocplx *q, ocplx *eval) {
-    const ocplx A = Z[iA], B = Z[iB], Cv = Z[iC], D = Z[iD], F = Z[iF], G = Z[iG], H = Z[iH];
-    const ocplx Ac = Z[iAc], Bc = Z[iBc], Cc = Z[iCc], Dc = Z[iDc], Fc = Z[iFc], Gc = Z[iGc],
-                Hc = Z[iHc];
-    const ocplx Z0 = Z[21];
-    const ocplx *P = &q[0], *Pc = &q[8];
-    const ocplx Z02 = cpow2(Z0), Z03 = cpow3(Z0);
-    for (int j = 1; j <= 7; ++j) {
-        const ocplx U = Z[13 + j], Q = q[15 + j], Qc = q[22 + j];
-        const ocplx Pj = P[j], Pcj = Pc[j], P0 = P[0], Pc0 = Pc[0];
-        /* I_j */
-        {
-            ocplx T1 = cneg(cmul(cadd(cmul(Pj, Pcj), cmul(P0, Pc0)), Z03));
-            ocplx T2 = cmul(cmul(cmul(csub(Cc, Ac), Qc), Z0), csub(cmul(Pj, Z0), A));
-            ocplx T3 = cmul(cmul(Ac, Z0), cadd(csub(Cv, A), cmul(Pj, Z0)));
-            ocplx T4 = cmul(cmul(Hc, Z0), csub(cmul(P0, Z0), Cv));
-            ocplx T5 = cmul(cmul(cmul(csub(Cv, A), Q), Z0), csub(cmul(Pcj, Z0), Ac));
-            ocplx T6 = cmul(cmul(A, Z0), cadd(csub(Cc, Ac), cmul(Pcj, Z0)));
-            ocplx T7 = cmul(cmul(U, csub(cmul(P0, Z0), H)),
-                            cadd(csub(cneg(Ac), cmul(csub(Cc, Ac), Qc)), cmul(Pcj, Z0)));
-            ocplx T8 = cmul(cmul(H, Z0), csub(cmul(Pc0, Z0), Cc));
-            ocplx T9 = cdiv(cmul(cmul(Z02, cadd(csub(cneg(A), cmul(csub(Cv, A), Q)), cmul(Pj, Z0))),
-                                 csub(cmul(Pc0, Z0), Hc)),
-                            U);
-            eval[j - 1] =
-                cadd(cadd(cadd(cadd(cadd(cadd(cadd(cadd(T1, T2), T3), T4), T5), T6), T7), T8), T9);
-        }
-        /* II_j */
-        {
-            ocplx T1 = cmul(cmul(Bc, Pj), Z02);
-            ocplx T2 = cmul(cmul(B, Pcj), Z02);
-            ocplx T3 = cmul(cadd(cmul(Pj, Pcj), cmul(P0, Pc0)), Z03);
-            ocplx T4 = cmul(cmul(Fc, Z0), csub(cmul(P0, Z0), B));
-            ocplx T5 = cmul(cmul(U, csub(cmul(P0, Z0), F)), csub(cmul(Pcj, Z0), Bc));
-            ocplx T6 = cmul(cmul(F, Z0), csub(cmul(Pc0, Z0), Bc));
-            ocplx T7 = cdiv(cmul(cmul(Z02, csub(cmul(Pj, Z0), B)), csub(cmul(Pc0, Z0), Fc)), U);
-            eval[j + 6] = cadd(cadd(cadd(cadd(csub(cadd(T1, T2), T3), T4), T5), T6), T7);
-        }
-        /* III_j */
-        {
-            ocplx E1 = cadd(csub(cadd(csub(A, B), cmul(csub(Cv, A), Q)),
-                                 cdiv(cmul(csub(D, B), csub(cmul(Pj, Z0), B)), csub(F, B))),
-                            cdiv(cmul(csub(G, Cv), cadd(csub(cneg(A), cmul(csub(Cv, A), Q)), cmul(Pj, Z0))),
-                                 csub(H, Cv)));
-            ocplx E1c = cadd(csub(cadd(csub(Ac, Bc), cmul(csub(Cc, Ac), Qc)),
-                                  cdiv(cmul(csub(Dc, Bc), csub(cmul(Pcj, Z0), Bc)), csub(Fc, Bc))),
-                             cdiv(cmul(csub(Gc, Cc), cadd(csub(cneg(Ac), cmul(csub(Cc, Ac), Qc)), cmul(Pcj, Z0))),
-                                  csub(Hc, Cc)));
-            ocplx Y1 = cadd(cneg(cdiv(cmul(csub(D, B), csub(cmul(P0, Z0), F)), csub(F, B))),
-                            cdiv(cmul(csub(G, Cv), csub(cmul(P0, Z0), H)), csub(H, Cv)));
-            ocplx Y1c = cadd(cneg(cdiv(cmul(csub(Dc, Bc), csub(cmul(Pc0, Z0), Fc)), csub(Fc, Bc))),
-                             cdiv(cmul(csub(Gc, Cc), csub(cmul(Pc0, Z0), Hc)), csub(Hc, Cc)));
-            ocplx X1 = cmul(cmul(csub(G, D), csub(Gc, Dc)), Z0);
-            ocplx X2 = cmul(cmul(Z0, E1), E1c);
-            ocplx X3 = cmul(cmul(U, Y1), E1c);
-            ocplx X4 = cdiv(cmul(cmul(Z02, E1), Y1c), U);
-            ocplx X5 = cmul(cmul(Z0, Y1), Y1c);
-            eval[j + 13] = csub(cadd(cadd(csub(X1, X2), X3), X4), X5);
-        }
-    }
+#define FS_NAME oracle_fsys
+#define FS_IN ocplx
+#define FS_OUT ocplx
+#define FS_T ocplx
+#define FS_LD(x) (x)
+#define FS_ST(x) (x)
+#define FS_ADD cadd
+#define FS_SUB csub
+#define FS_NEG cneg
+#define FS_MUL cmul
+#define FS_DIV cdiv
+#define FS_P2 cpow2
+#define FS_P3 cpow3
+#include "fsys_expanded.inc"
+#undef FS_NAME
+#undef FS_IN
+#undef FS_OUT
+#undef FS_T
+#undef FS_LD
+#undef FS_ST
+#undef FS_ADD
+#undef FS_SUB
+#undef FS_NEG
+#undef FS_MUL
+#undef FS_DIV
+#undef FS_P2
+#undef FS_P3
+
+/* The same residuals in IEEE binary128 (gcc's __float128 complex, software arithmetic): the
+ * extended-precision residual of the final refinement.  Inputs are doubles, taken exactly. */
+typedef _Complex float __attribute__((mode(TC))) qcplx;
+static inline qcplx q_of(ocplx z) {
+    qcplx r;
+    __real__ r = (__float128)z.re;
+    __imag__ r = (__float128)z.im;
+    return r;
 }
+static inline ocplx o_of(qcplx z) { return C((double)__real__ z, (double)__imag__ z); }
+static inline qcplx qadd(qcplx a, qcplx b) { return a + b; }
+static inline qcplx qsub(qcplx a, qcplx b) { return a - b; }
+static inline qcplx qneg(qcplx a) { return -a; }
+static inline qcplx qmul(qcplx a, qcplx b) { return a * b; }
+static inline qcplx qdiv(qcplx a, qcplx b) { return a / b; }
+static inline qcplx qp2(qcplx a) { return a * a; }
+static inline qcplx qp3(qcplx a) { return a * a * a; }
+#define FS_NAME fsys_quad
+#define FS_IN ocplx
+#define FS_OUT ocplx
+#define FS_T qcplx
+#define FS_LD q_of
+#define FS_ST o_of
+#define FS_ADD qadd
+#define FS_SUB qsub
+#define FS_NEG qneg
+#define FS_MUL qmul
+#define FS_DIV qdiv
+#define FS_P2 qp2
+#define FS_P3 qp3
+static
+#include "fsys_expanded.inc"
+#undef FS_NAME
+#undef FS_IN
+#undef FS_OUT
+#undef FS_T
+#undef FS_LD
+#undef FS_ST
+#undef FS_ADD
+#undef FS_SUB
+#undef FS_NEG
+#undef FS_MUL
+#undef FS_DIV
+#undef FS_P2
+#undef FS_P3
+
+void oracle_fsys_quad(const ocplx *Z, const ocplx *q, ocplx *F) { fsys_quad(Z, q, F); }
 
 /* All three equation families share one shape (derived from the expanded forms above):
  *     F = Z0*K - W*Wc,   W = Z0*X - U*Y,   Wc = Xc - (Z0/U)*Yc,
@@ -940,11 +955,57 @@ void oracle_cognate_solns(const ocplx *z, const ocplx *q, ocplx *out) {
 /* L6: the Homotopy kernel body (frg.cu:35-69)                                                 */
 /* ------------------------------------------------------------------------------------------ */
 
-static void finish_path(const ocplx *fpm, ocplx *Yf, ocplx *fpt) {
+/* A refinement step longer than this fraction of max(1, |Y|) is discarded and the refinement
+ * ends: the point is not in Newton's quadratic basin. */
+#define POLISH_CAP 1e-6
+
+/* Final FP64 refinement of a code-0 root (not in the reference; the north star's "final root
+ * refinement"): polish_its Newton steps at t = 1 in the dehomogenising patch e_21 that
+ * SwitchPatch has just applied (frg.cu:44-46,60), with Z0 = 1 exactly, so the root is refined in
+ * the coordinates it is reported in.  Each step is iterative refinement with an extended
+ * residual: the Jacobian, the row scaling and the solve are the double-precision Newton step of
+ * htpysupport.cu:430-480 (JH with start = target so that q(1) = fpm exactly, LinearSolveNewton
+ * with dt = 0), while F is evaluated in binary128 and rounded, so the iteration converges to the
+ * double nearest the root even when the root is ill-conditioned.  A step is taken only while it
+ * is no longer than POLISH_CAP * max(1, |Y|). */
+static void polish_root(const otrk_opts *o, const ocplx *u_dehom, const ocplx *fpm, const ocplx *gamma, ocplx *Y,
+                        otrk_stats *st) {
+    Y[ONV] = C(1., 0);
+    for (int k = 0; k < o->polish_its; ++k) {
+        ocplx M[ONY * ONY], Hv[ONV + 1], b[ONV + 1], d[ONY];
+        double scales[ONV + 1];
+        oracle_JH(Y, u_dehom, fpm, fpm, gamma, M);
+        for (int j = 0; j < ONY; ++j) M[(ONV + 1) * ONY + j] = C(0, 0); /* unused row */
+        fsys_quad(Y, fpm, Hv);
+        Hv[ONV] = csub(vdot(u_dehom, Y, ONY), C(1, 0));
+        scale_rows(M, scales);
+        for (int j = 0; j < ONV + 1; ++j) b[j] = C(-scales[j] * Hv[j].re, -scales[j] * Hv[j].im);
+        oracle_linear_solve_newton(M, b, 0.0, d);
+        const double r = vnorm(d, ONY);
+        if (!(r <= POLISH_CAP * fmax(1.0, vnorm(Y, ONY)))) break;
+        for (int j = 0; j < ONY; ++j) Y[j] = cadd(Y[j], d[j]);
+        if (st) st->newton_its++;
+    }
+}
+
+int oracle_refine_root(const otrk_opts *o, const ocplx *fpm, ocplx gamma, ocplx *Y) {
+    ocplx u_dehom[ONY];
+    memset(u_dehom, 0, sizeof(u_dehom));
+    u_dehom[ONV] = C(1., 0);
+    otrk_stats st;
+    memset(&st, 0, sizeof(st));
+    polish_root(o, u_dehom, fpm, &gamma, Y, &st);
+    return st.newton_its;
+}
+
+/* Dehomogenise (frg.cu:60), polish code-0 roots, take the canonical cognate (frg.cu:63-68). */
+static void finish_path(const otrk_opts *o, int code, const ocplx *fpm, const ocplx *gamma, ocplx *Yf, ocplx *fpt,
+                        otrk_stats *st) {
     ocplx u_dehom[ONY];
     memset(u_dehom, 0, sizeof(u_dehom));
     u_dehom[ONV] = C(1., 0);
     oracle_switch_patch(u_dehom, Yf, Yf);
+    if (code == 0 && o->polish_its > 0) polish_root(o, u_dehom, fpm, gamma, Yf, st);
     ocplx cogn[2 * ONV];
     oracle_cognate_solns(Yf, fpm, cogn);
     memcpy(fpt, cogn, sizeof(ocplx) * ONV);
@@ -972,9 +1033,9 @@ int oracle_homotopy(const otrk_opts *o, const ocplx *fpm, ocplx gamma, uint64_t 
         uint64_t pid = first_path_id + (uint64_t)i;
         oracle_generate_start(o, seed, pid, spt, spm);
         err[i] = oracle_track_path(o, spt, spm, fpm, &gamma, seed, pid, 0, Yf, &st);
-        if (stats) stats[i] = st;
         if (yfinal) memcpy(&yfinal[(int64_t)i * ONY], Yf, sizeof(Yf));
-        finish_path(fpm, Yf, &fpt[(int64_t)i * ONV]);
+        finish_path(o, err[i], fpm, &gamma, Yf, &fpt[(int64_t)i * ONV], &st);
+        if (stats) stats[i] = st;
     }
     return 0;
 }
@@ -992,9 +1053,25 @@ int oracle_homotopy_explicit(const otrk_opts *o, const ocplx *fpm, ocplx gamma, 
         uint64_t pid = first_path_id + (uint64_t)i;
         err[i] = oracle_track_path(o, &spt[(int64_t)i * ONV], &spm[(int64_t)i * ONP], fpm, &gamma,
                                    seed, pid, 0, Yf, &st);
-        if (stats) stats[i] = st;
         if (yfinal) memcpy(&yfinal[(int64_t)i * ONY], Yf, sizeof(Yf));
-        finish_path(fpm, Yf, &fpt[(int64_t)i * ONV]);
+        finish_path(o, err[i], fpm, &gamma, Yf, &fpt[(int64_t)i * ONV], &st);
+        if (stats) stats[i] = st;
+    }
+    return 0;
+}
+
+int oracle_homotopy_explicit_ids(const otrk_opts *o, const ocplx *fpm, ocplx gamma, const ocplx *spt,
+                                 const ocplx *spm, uint64_t seed, const uint64_t *path_ids, int32_t n,
+                                 ocplx *fpt, int32_t *err, int nthreads) {
+    int nt = set_threads(nthreads);
+    (void)nt;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+    for (int32_t i = 0; i < n; ++i) {
+        ocplx Yf[ONY];
+        otrk_stats st;
+        err[i] = oracle_track_path(o, &spt[(int64_t)i * ONV], &spm[(int64_t)i * ONP], fpm, &gamma, seed,
+                                   path_ids[i], 0, Yf, &st);
+        finish_path(o, err[i], fpm, &gamma, Yf, &fpt[(int64_t)i * ONV], &st);
     }
     return 0;
 }

@@ -48,6 +48,10 @@ typedef struct {
     double start_lo, start_hi;
     int32_t start_mode;     /* 0: U[start_lo,start_hi]^2 (reference), 1: standard complex Gaussian */
     int32_t sharp_residual; /* 1: |Ydiff| (reference), 0: |H| */
+    int32_t polish_its;     /* Newton iterations on a code-0 root in the dehomogenising patch e_21
+                               (frg.cu:44-46) at t = 1 after sharpening; 0 = none, as the
+                               reference.  Not in the reference: the final FP64 refinement. */
+    int32_t pad0;
 } otrk_opts;
 
 /* Per-path counters (not in the reference; used for FLOP accounting and histograms). */
@@ -75,6 +79,8 @@ void oracle_csqrt(const ocplx *x, ocplx *out);
 
 /* ---- L1 system (htpykinematics.cu) ---- */
 void oracle_fsys(const ocplx *Z /*22*/, const ocplx *q /*30*/, ocplx *F /*21*/);
+/* FSYS evaluated in IEEE binary128 and rounded to double (the refinement's residual). */
+void oracle_fsys_quad(const ocplx *Z /*22*/, const ocplx *q /*30*/, ocplx *F /*21*/);
 void oracle_jvar(const ocplx *Z, const ocplx *q, ocplx *J /*21x22 row-major*/);
 void oracle_jpar(const ocplx *Z, const ocplx *q, ocplx *J /*21x30 row-major*/);
 
@@ -104,6 +110,10 @@ int oracle_track_path(const otrk_opts *o, const ocplx *spt, const ocplx *spm, co
                       const ocplx *gamma, uint64_t seed, uint64_t path_id, uint64_t first_draw,
                       ocplx *Yfinal /*23*/, otrk_stats *st);
 
+/* The final refinement alone, on a dehomogenised point Y (23: z, Z0 := 1, t = 1): polish_its
+ * Newton steps in the patch e_21 with the binary128 residual.  Returns the steps taken. */
+int oracle_refine_root(const otrk_opts *o, const ocplx *fpm /*30*/, ocplx gamma, ocplx *Y /*23*/);
+
 /* ---- L5 start system and cognates (frgkinematics.cu) ---- */
 void oracle_generate_start(const otrk_opts *o, uint64_t seed, uint64_t path_id, ocplx *z /*21*/,
                            ocplx *q /*30*/);
@@ -121,6 +131,12 @@ int oracle_homotopy_explicit(const otrk_opts *o, const ocplx *fpm, ocplx gamma, 
                              const ocplx *spm, uint64_t seed, uint64_t first_path_id, int32_t n,
                              ocplx *fpt, int32_t *err, otrk_stats *stats, ocplx *yfinal,
                              int nthreads);
+
+/* Explicit starts with an RNG path id per path (patch draws of path i from path_ids[i]): many
+ * copies of one path under perturbed start data in one parallel call. */
+int oracle_homotopy_explicit_ids(const otrk_opts *o, const ocplx *fpm, ocplx gamma, const ocplx *spt,
+                                 const ocplx *spm, uint64_t seed, const uint64_t *path_ids, int32_t n,
+                                 ocplx *fpt, int32_t *err, int nthreads);
 
 /* ---- L7 host post-processing (frgsupport.cu) ---- */
 double oracle_lambertw(double z);

@@ -38,6 +38,7 @@ class Opts(ct.Structure):
         ("patch_lo", ct.c_double), ("patch_hi", ct.c_double),
         ("start_lo", ct.c_double), ("start_hi", ct.c_double),
         ("start_mode", ct.c_int32), ("sharp_residual", ct.c_int32),
+        ("polish_its", ct.c_int32), ("pad0", ct.c_int32),
     ]
 
 
@@ -63,7 +64,10 @@ def lib(fast: bool = False) -> ct.CDLL:
         for fn in ("oracle_cdiv",):
             getattr(L, fn).argtypes = [P, P, P]
         L.oracle_csqrt.argtypes = [P, P]
-        for fn in ("oracle_fsys", "oracle_jvar", "oracle_jpar"):
+        L.oracle_refine_root.argtypes = [ct.POINTER(Opts), P, CCplx, P]
+        L.oracle_homotopy_explicit_ids.argtypes = [ct.POINTER(Opts), P, CCplx, P, P, ct.c_uint64, P, ct.c_int32,
+                                                   P, P, ct.c_int]
+        for fn in ("oracle_fsys", "oracle_fsys_quad", "oracle_jvar", "oracle_jpar"):
             getattr(L, fn).argtypes = [P, P, P]
         L.oracle_switch_patch.argtypes = [P, P, P]
         for fn in ("oracle_q", "oracle_dqdt"):
@@ -147,6 +151,36 @@ def csqrt(x):
 def fsys(Z, q):
     Z, q, o = _c(Z, 22), _c(q, NP), np.zeros(NV, np.complex128)
     lib().oracle_fsys(_p(Z), _p(q), _p(o))
+    return o
+
+
+def homotopy_explicit_ids(opts, fpm, gamma, spt, spm, seed, path_ids, nthreads=0):
+    """Explicit starts, path i drawing its patches from RNG id path_ids[i]; returns (fpt, err)."""
+    spt = np.ascontiguousarray(spt, np.complex128)
+    spm = np.ascontiguousarray(spm, np.complex128)
+    n = spt.shape[0]
+    ids = np.ascontiguousarray(path_ids, np.uint64)
+    assert spt.shape == (n, NV) and spm.shape == (n, NP) and ids.shape == (n,)
+    fpt = np.zeros((n, NV), np.complex128)
+    err = np.zeros(n, np.int32)
+    lib().oracle_homotopy_explicit_ids(ct.byref(opts), _p(_c(fpm, NP)), _cd(gamma), _p(spt), _p(spm), seed,
+                                       _p(ids), n, _p(fpt), _p(err), int(nthreads))
+    return fpt, err
+
+
+def refine_root(opts, fpm, gamma, Y):
+    """The final refinement of a dehomogenised point Y (23); returns (refined Y, steps taken)."""
+    Y = _c(Y, 23).copy()
+    k = lib().oracle_refine_root(ct.byref(opts), _p(_c(fpm, 30)), _cd(gamma), _p(Y))
+    return Y, k
+
+
+def fsys_quad(Z, q):
+    """FSYS evaluated in IEEE binary128 and rounded to double (the refinement's residual)."""
+    Z = _c(Z, 22)
+    q = _c(q, 30)
+    o = np.zeros(21, complex)
+    lib().oracle_fsys_quad(_p(Z), _p(q), _p(o))
     return o
 
 

@@ -15,13 +15,16 @@ import numpy as np
 
 
 def fsys(Z, q):
-    Z = np.atleast_2d(np.asarray(Z, complex))
-    q = np.atleast_2d(np.asarray(q, complex))
+    """Residuals F (batch x 21).  Object arrays (mpmath mpc entries) are evaluated in their own
+    arithmetic, for extended-precision references."""
+    dt = object if (np.asarray(Z).dtype == object or np.asarray(q).dtype == object) else complex
+    Z = np.atleast_2d(np.asarray(Z, dt))
+    q = np.atleast_2d(np.asarray(q, dt))
     A, B, C, D, F, G, H = (Z[:, i] for i in range(7))
     Ac, Bc, Cc, Dc, Fc, Gc, Hc = (Z[:, i] for i in range(7, 14))
     Z0 = Z[:, 21]
     P0, Pc0 = q[:, 0], q[:, 8]
-    out = np.zeros((Z.shape[0], 21), complex)
+    out = np.zeros((Z.shape[0], 21), dt)
     for j in range(1, 8):
         U = Z[:, 13 + j]
         Pj, Pcj, Qj, Qcj = q[:, j], q[:, 8 + j], q[:, 15 + j], q[:, 22 + j]

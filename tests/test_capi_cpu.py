@@ -32,6 +32,8 @@ def test_defaults_match_reference_constants():
     o = capi.default_opts()
     r = O.default_opts()
     for f, _ in O.Opts._fields_:
+        if f == "pad0":
+            continue
         if f == "sharp_residual":  # the ABI has one residual type for both Newton uses
             assert r.sharp_residual == o.newt_residual
             continue

@@ -8,6 +8,7 @@ import pytest
 
 from finite_root_generation_amd import capi, tasks
 from oracle import mlp_oracle as M
+from tests.rootcmp import ROOT_TOL, abs_diff
 
 pytestmark = pytest.mark.gpu
 
@@ -100,8 +101,8 @@ def test_learned_control_gain_on_held_out_task(tracker, records):
     (fa, ea, sa, _), (fb, eb, sb, _) = res["classical"], res["learned"]
     ma, mb = tasks.finite_mask(fa, ea), tasks.finite_mask(fb, eb)
     both = ma & mb
-    d = np.max(np.abs(fb[both] - fa[both]) / np.maximum(1.0, np.abs(fa[both])), axis=1)
-    recall = (d <= 1e-8).sum() / ma.sum()
+    d = abs_diff(fb[both], fa[both])
+    recall = (d <= ROOT_TOL).sum() / ma.sum()
     rk_a, rk_b = sa["rk_attempts"].mean(), sb["rk_attempts"].mean()
     solves_a = (sa["tangent_solves"] + sa["newton_its"]).mean()
     solves_b = (sb["tangent_solves"] + sb["newton_its"]).mean()

@@ -7,6 +7,7 @@ import pytest
 
 from finite_root_generation_amd import capi, tasks
 from oracle import oracle as O
+from tests.rootcmp import ROOT_TOL, abs_diff
 from tests.test_gpu_parity import _starts, rounding_sensitive
 
 pytestmark = pytest.mark.gpu
@@ -56,8 +57,8 @@ def test_limits_match_oracle(opts, code):
         assert np.all((err_g == code) | np.isin(np.arange(N), bad))
     ok = (err_g == err_o) & tasks.finite_mask(fpt_o, err_o)
     if ok.any():
-        d = np.max(np.abs(fpt_g[ok] - fpt_o[ok]) / np.maximum(1.0, np.abs(fpt_o[ok])), axis=1)
-        assert np.mean(d <= 1e-8) >= 0.95
+        d = abs_diff(fpt_g[ok], fpt_o[ok])
+        assert np.mean(d <= ROOT_TOL) >= 0.95
     for f in ("steps", "rk_attempts", "redraws"):
         assert np.mean(st_g[f] == st_o[f]) >= 0.5, f
 

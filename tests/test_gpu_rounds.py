@@ -12,6 +12,7 @@ import pytest
 
 from finite_root_generation_amd import capi, tasks
 from tests import stephenson_np as S
+from tests.rootcmp import ROOT_TOL, abs_diff
 
 pytestmark = pytest.mark.gpu
 
@@ -99,8 +100,7 @@ def test_rounds_with_learned_step_control(tracker, tmp_path):
     ecoll, _ = expected(tracker, 1)
     found = 0
     for r in ecoll:
-        d = np.max(np.abs(coll - r[None]) / np.maximum(1.0, np.abs(r[None])), axis=1)
-        found += bool(d.min() <= 1e-8)
+        found += bool(abs_diff(coll, r[None]).min() <= ROOT_TOL)
     say(f"    learned collection {coll.shape[0]} roots, plain {ecoll.shape[0]}; plain roots found {found}")
     assert found >= 0.97 * ecoll.shape[0]
 
@@ -112,5 +112,25 @@ def test_rounds_per_launch_gives_the_same_files(tracker, tmp_path):
     coll, codes, progress = read_files(tmp_path)
     ecoll, ecodes = expected(tracker, 3)
     assert np.array_equal(codes, ecodes)
+    assert np.array_equal(coll.view(np.float64), ecoll.view(np.float64))
+    assert progress.count("-----Progress statistics for round ") == 3
+
+
+def test_resume_with_another_path_count_uses_fresh_ids(tracker, tmp_path):
+    """A resumed run with a different --paths continues the path ids after the trials already
+    recorded in errorCodesAgr, so no path is tracked twice: the files equal those of rounds over
+    consecutive ids (N, then two rounds of M)."""
+    m = N // 2 + 100
+    run_rounds(tmp_path, 1)
+    run_rounds(tmp_path, 2, "--paths", str(m))
+    coll, codes, progress = read_files(tmp_path)
+    assert codes.shape == (N + 2 * m,)
+    ecoll = np.zeros((0, 21), np.complex128)
+    ecodes = []
+    for first, n in ((0, N), (N, m), (N + m, m)):
+        fpt, err, _, _ = tracker.track(tasks.REF_FPM, tasks.REF_GAMMA, seed=1, n=n, first_path_id=first)
+        ecoll, err = capi.sort_dedupe_merge(ecoll, fpt, err, 1e-6)
+        ecodes.append(err)
+    assert np.array_equal(codes, np.concatenate(ecodes))
     assert np.array_equal(coll.view(np.float64), ecoll.view(np.float64))
     assert progress.count("-----Progress statistics for round ") == 3
