@@ -14,35 +14,39 @@ Vectorised over a leading batch axis.
 import numpy as np
 
 
+def pose_shapes(Z, q, j):
+    """(X, Y, Xc, Yc, K) of equations I_j, II_j, III_j (pose j = 1..7), batch-vectorised; each
+    equation is F = Z0*K - (Z0*X - U*Y) * (Xc - (Z0/U)*Yc)."""
+    A, B, C, D, F, G, H = (Z[:, i] for i in range(7))
+    Ac, Bc, Cc, Dc, Fc, Gc, Hc = (Z[:, i] for i in range(7, 14))
+    Z0 = Z[:, 21]
+    P0, Pc0 = q[:, 0], q[:, 8]
+    Pj, Pcj, Qj, Qcj = q[:, j], q[:, 8 + j], q[:, 15 + j], q[:, 22 + j]
+    eq1 = (Pj * Z0 - A - (C - A) * Qj, P0 * Z0 - H, Pcj * Z0 - Ac - (Cc - Ac) * Qcj, Pc0 * Z0 - Hc,
+           Ac * C - Ac * A - Hc * C + A * Cc - H * Cc + H * Hc + (C - A) * (Cc - Ac) * Qj * Qcj)
+    eq2 = (Pj * Z0 - B, P0 * Z0 - F, Pcj * Z0 - Bc, Pc0 * Z0 - Fc, (B - F) * (Bc - Fc))
+    r1, r2 = (D - B) / (F - B), (G - C) / (H - C)
+    r1c, r2c = (Dc - Bc) / (Fc - Bc), (Gc - Cc) / (Hc - Cc)
+    X = (A - B) + (C - A) * Qj - r1 * (Pj * Z0 - B) + r2 * (Pj * Z0 - A - (C - A) * Qj)
+    Y = -r1 * (P0 * Z0 - F) + r2 * (P0 * Z0 - H)
+    Xc = (Ac - Bc) + (Cc - Ac) * Qcj - r1c * (Pcj * Z0 - Bc) + r2c * (Pcj * Z0 - Ac - (Cc - Ac) * Qcj)
+    Yc = -r1c * (Pc0 * Z0 - Fc) + r2c * (Pc0 * Z0 - Hc)
+    eq3 = (X, Y, Xc, Yc, (G - D) * (Gc - Dc))
+    return eq1, eq2, eq3
+
+
 def fsys(Z, q):
     """Residuals F (batch x 21).  Object arrays (mpmath mpc entries) are evaluated in their own
     arithmetic, for extended-precision references."""
     dt = object if (np.asarray(Z).dtype == object or np.asarray(q).dtype == object) else complex
     Z = np.atleast_2d(np.asarray(Z, dt))
     q = np.atleast_2d(np.asarray(q, dt))
-    A, B, C, D, F, G, H = (Z[:, i] for i in range(7))
-    Ac, Bc, Cc, Dc, Fc, Gc, Hc = (Z[:, i] for i in range(7, 14))
     Z0 = Z[:, 21]
-    P0, Pc0 = q[:, 0], q[:, 8]
     out = np.zeros((Z.shape[0], 21), dt)
     for j in range(1, 8):
         U = Z[:, 13 + j]
-        Pj, Pcj, Qj, Qcj = q[:, j], q[:, 8 + j], q[:, 15 + j], q[:, 22 + j]
-
-        def shape(X, Y, Xc, Yc, K):
-            return Z0 * K - (Z0 * X - U * Y) * (Xc - (Z0 / U) * Yc)
-
-        out[:, j - 1] = shape(Pj * Z0 - A - (C - A) * Qj, P0 * Z0 - H,
-                              Pcj * Z0 - Ac - (Cc - Ac) * Qcj, Pc0 * Z0 - Hc,
-                              Ac * C - Ac * A - Hc * C + A * Cc - H * Cc + H * Hc + (C - A) * (Cc - Ac) * Qj * Qcj)
-        out[:, 6 + j] = shape(Pj * Z0 - B, P0 * Z0 - F, Pcj * Z0 - Bc, Pc0 * Z0 - Fc, (B - F) * (Bc - Fc))
-        r1, r2 = (D - B) / (F - B), (G - C) / (H - C)
-        r1c, r2c = (Dc - Bc) / (Fc - Bc), (Gc - Cc) / (Hc - Cc)
-        X = (A - B) + (C - A) * Qj - r1 * (Pj * Z0 - B) + r2 * (Pj * Z0 - A - (C - A) * Qj)
-        Y = -r1 * (P0 * Z0 - F) + r2 * (P0 * Z0 - H)
-        Xc = (Ac - Bc) + (Cc - Ac) * Qcj - r1c * (Pcj * Z0 - Bc) + r2c * (Pcj * Z0 - Ac - (Cc - Ac) * Qcj)
-        Yc = -r1c * (Pc0 * Z0 - Fc) + r2c * (Pc0 * Z0 - Hc)
-        out[:, 13 + j] = shape(X, Y, Xc, Yc, (G - D) * (Gc - Dc))
+        for e, (X, Y, Xc, Yc, K) in enumerate(pose_shapes(Z, q, j)):
+            out[:, 7 * e + j - 1] = Z0 * K - (Z0 * X - U * Y) * (Xc - (Z0 / U) * Yc)
     return out
 
 

@@ -480,3 +480,50 @@ def test_refinement_reaches_the_nearest_double():
     print(f"refined {worst_ref:.2e} sharpened {worst_unref:.2e}")
     assert worst_ref < 8 * 2.0 ** -52, worst_ref
     assert worst_unref > 1e-12, worst_unref
+
+
+def test_no_rotation_makes_reference_cognate34_a_root():
+    """Closes the reference's disabled 3/4 cognate (frgkinematics.cu:140-182, 244-266) for every
+    choice of the rotations U_j, not only the one its commented-out code assigns.  At Z0 = 1 each
+    equation of pose j is F = K - (X - U Y)(Xc - Yc/U), so U F is the quadratic
+    (Y Xc) U^2 + (K - X Xc - Y Yc) U + X Yc in U.  A U that solves all three equations of the pose
+    is a common root of the three quadratics, so it is among their six roots: checking those six
+    candidates is complete.  Positive control: for the tracked root itself the three quadratics
+    share its U_j (relative residual ~1e-15); for the 3/4 cognate's linkage no candidate brings all
+    three equations below 1e-4 (relative), for any pose of any root."""
+    fpm, gamma = tasks.REF_FPM, tasks.REF_GAMMA
+    fpt, err, _, _ = O.homotopy(O.default_opts(), fpm, gamma, 1, 0, 24, nthreads=4)
+
+    def c34(o):
+        A, B, C, D, F, G, H = o
+        s = (H - G) / (C - G)
+        Ap, Bp, Cp, Gp = s * (A - D) + D, s * (B - D) + D, s * (C - D) + D, D + (H - G)
+        s2 = (B - F) / (Bp - F)
+        return np.array([s2 * (Ap - F) + F, B, s2 * (Cp - F) + F, s2 * (D - F) + F, F, s2 * (Gp - F) + F,
+                         s2 * (H - F) + F])
+
+    def best_common_rotation(link, j):
+        """min over the six candidate U of the largest relative residual of the three equations."""
+        Z = np.concatenate([link, np.zeros(7), [1.0]])[None, :]
+        quads = []
+        for X, Y, Xc, Yc, K in S.pose_shapes(Z, fpm[None, :], j):
+            quads.append((Y[0] * Xc[0], K[0] - X[0] * Xc[0] - Y[0] * Yc[0], X[0] * Yc[0]))
+        best = np.inf
+        for a, b, c in quads:
+            for u in np.roots([a, b, c]):
+                worst = max(abs(qa * u * u + qb * u + qc) / (abs(qa) * abs(u) ** 2 + abs(qb) * abs(u) + abs(qc))
+                            for qa, qb, qc in quads)
+                best = min(best, worst)
+        return best
+
+    ctrl, c34_best = [], []
+    for i in np.nonzero(err == 0)[0][:12]:
+        z = fpt[i]
+        link34 = np.concatenate([c34(z[0:7]), c34(z[7:14])])
+        for j in range(1, 8):
+            ctrl.append(best_common_rotation(z[:14], j))
+            c34_best.append(best_common_rotation(link34, j))
+    print(f"root: worst {max(ctrl):.1e}; 3/4 cognate: best {min(c34_best):.1e}, median {np.median(c34_best):.1e}")
+    assert len(ctrl) >= 70
+    assert max(ctrl) < 1e-10
+    assert min(c34_best) > 1e-4
