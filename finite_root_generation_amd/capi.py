@@ -24,7 +24,7 @@ SCHED_STATIC, SCHED_PERSISTENT = 0, 1
 STEP_CLASSIC, STEP_LEARNED = 0, 1
 
 ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "out of memory",
-          -4: "unsupported option"}
+          -4: "unsupported option", -5: "size beyond the fixed limit of the entry point"}
 
 PATH_STATS_FIELDS = ["steps", "rk_attempts", "newton_calls", "newton_its", "ministeps", "redraws",
                      "tangent_solves", "fp32_solves"]
@@ -35,7 +35,7 @@ EXPORTS = ["frg_default_opts", "frg_create", "frg_destroy", "frg_set_opts", "frg
            "frg_track_explicit", "frg_track_batch", "frg_generate_starts", "frg_solve_step",
            "frg_sort_dedupe_merge", "frg_lambertw", "frg_completeness", "frg_strerror", "frg_version",
            "frg_last_kernel_ms", "frg_last_refine_ms", "frg_flops", "frg_collect_step_log", "frg_train_step_model",
-           "frg_set_step_model", "frg_step_model_eval", "frg_step_model_grad", "frg_get_opts", "frg_autotune", "frg_sort_dedupe_merge_device"]
+           "frg_set_step_model", "frg_step_model_eval", "frg_step_model_grad", "frg_get_opts", "frg_autotune", "frg_sort_dedupe_merge_device", "frg_track_batch_noalloc"]
 STEP_MODEL_SIZE, STEP_MODEL_WEIGHTS, STEP_RECORD = 4819, 4801, 9
 
 
@@ -146,6 +146,7 @@ def lib() -> ct.CDLL:
         L.frg_version.restype = ct.c_int
         L.frg_last_kernel_ms.argtypes = [P, ct.POINTER(ct.c_double)]
         L.frg_last_refine_ms.argtypes = [P, ct.POINTER(ct.c_double)]
+        L.frg_track_batch_noalloc.argtypes = [P, ct.c_int32, P, P, P, P, P, P, P, P, P]
         L.frg_get_opts.argtypes = [P, ct.POINTER(Opts)]
         L.frg_sort_dedupe_merge_device.argtypes = [P, P, ct.c_int64, P, P, ct.c_int64, ct.c_double, P,
                                                    ct.POINTER(ct.c_int64), P]

@@ -71,6 +71,8 @@ struct frg_ctx {
         int kind, block, grid;
     } graph_key;
     bool graph_valid = false;
+    // this GPU's fixed scratch array of the allocation-free entry point (device address)
+    void *noalloc_scratch = nullptr;
 };
 
 namespace {
@@ -151,7 +153,9 @@ int check_opts(const frg_opts &o) {
 // to the occupancy the kernel's register use allows.
 int pick_block(const frg_ctx *c) { return c->opts.block_size ? c->opts.block_size : 64; }
 
-int run_job(frg_ctx *c, frg::Job &job, hipStream_t stream, frg_stats *summary) {
+// allow_graph = false launches eagerly whatever opts.use_graph says (no graph is created); a job
+// that brings its own work-queue counter (job.queue) uses it instead of the context's.
+int run_job(frg_ctx *c, frg::Job &job, hipStream_t stream, frg_stats *summary, bool allow_graph = true) {
     job.o = to_track_opts(c->opts);
     job.learned = c->opts.step_control == FRG_STEP_LEARNED;
     if (job.learned && !c->have_model) return FRG_EINVAL;
@@ -167,15 +171,17 @@ int run_job(frg_ctx *c, frg::Job &job, hipStream_t stream, frg_stats *summary) {
         grid = std::max(1, (per_cu / gpb) * c->num_cu);
         const int64_t need = (job.n_total + gpb - 1) / gpb;
         if ((int64_t)grid > need) grid = (int)need;
-        HIPCHK(c->queue.ensure(sizeof(unsigned long long)));
-        job.queue = c->queue.as<unsigned long long>();
+        if (!job.queue) {
+            HIPCHK(c->queue.ensure(sizeof(unsigned long long)));
+            job.queue = c->queue.as<unsigned long long>();
+        }
     } else {
         const int64_t g = (job.n_total + gpb - 1) / gpb;
         if (g > 0x7fffffff) return FRG_EINVAL;
         grid = (int)g;
     }
     if (job.n_total == 0) return FRG_OK;
-    if (c->opts.use_graph) {
+    if (c->opts.use_graph && allow_graph) {
         // one graph per launch signature: the kernel arguments (the Job) and the launch shape
         frg_ctx::GraphKey key;
         std::memset(&key, 0, sizeof(key));
@@ -340,6 +346,7 @@ const char *frg_strerror(int code) {
         case FRG_EHIP: return "HIP runtime error";
         case FRG_ENOMEM: return "out of memory";
         case FRG_EUNSUPPORTED: return "unsupported option";
+        case FRG_ELIMIT: return "size beyond the fixed limit of the entry point";
         default: return "unknown error";
     }
 }
@@ -407,6 +414,12 @@ int frg_create(frg_ctx **out, int device, const frg_opts *opts) {
         return FRG_EHIP;
     }
     c->num_cu = prop.multiProcessorCount;
+    // the allocation-free entry point's scratch array: its address on this device, looked up once
+    // with the device current
+    if (frg::noalloc_scratch_address(&c->noalloc_scratch) != hipSuccess || !c->noalloc_scratch) {
+        frg_destroy(c);
+        return FRG_EHIP;
+    }
     *out = c;
     return FRG_OK;
 }
@@ -506,6 +519,47 @@ int frg_track_batch(frg_ctx *c, int32_t n_inst, const frg_c64 *fpm, const frg_c6
     if (rc != FRG_OK) return rc;
     if (job.n_total > 0 && (!fpt || !err)) return FRG_EINVAL;
     return run_host(c, job, fpt, err, path_stats, nullptr, summary, t0);
+}
+
+int frg_track_batch_noalloc(frg_ctx *c, int32_t n_inst, const frg_c64 *d_fpm, const frg_c64 *gamma,
+                            const uint64_t *seeds, const uint64_t *first_path_ids, const int64_t *paths_per_inst,
+                            frg_c64 *d_fpt, int32_t *d_err, int32_t *d_path_stats, void *stream) {
+    if (!c || n_inst < 1 || !d_fpm || !gamma || !seeds || !first_path_ids || !paths_per_inst) return FRG_EINVAL;
+    if (n_inst > FRG_NOALLOC_MAX_INSTANCES) return FRG_ELIMIT;
+    if (!c->noalloc_scratch) return FRG_EHIP;
+    frg::NoAllocStage st;  // kernel arguments of the staging kernel (host stack, no allocation)
+    st.n_inst = n_inst;
+    st.inst_first[0] = 0;
+    for (int i = 0; i < n_inst; ++i) {
+        if (paths_per_inst[i] < 0) return FRG_EINVAL;
+        st.inst_first[i + 1] = st.inst_first[i] + paths_per_inst[i];
+        st.gamma[i] = frg::mk(gamma[i].re, gamma[i].im);
+        st.seed[i] = seeds[i];
+        st.pid0[i] = first_path_ids[i];
+    }
+    const int64_t total = st.inst_first[n_inst];
+    if (total > 0 && (!d_fpt || !d_err)) return FRG_EINVAL;
+    BEGIN(c);
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    const frg::NoAllocLayout L = frg::noalloc_layout(static_cast<unsigned char *>(c->noalloc_scratch));
+    frg::Job job{};
+    job.n_total = total;
+    job.n_inst = n_inst;
+    job.inst_first = L.inst_first;
+    job.fpm = reinterpret_cast<const cx<double> *>(d_fpm);
+    job.gamma = L.gamma;
+    job.seed = L.seed;
+    job.pid0 = L.pid0;
+    job.fpt = reinterpret_cast<cx<double> *>(d_fpt);
+    job.err = d_err;
+    job.stats = d_path_stats;
+    job.yfinal = nullptr;
+    job.queue = L.queue;
+    HIPCHK(frg::launch_stage_instances(st, c->noalloc_scratch, s));
+    const int rc = run_job(c, job, s, nullptr, /*allow_graph=*/false);
+    if (rc != FRG_OK) return rc;
+    HIPCHK(hipEventRecord(c->ev_busy, s));
+    return FRG_OK;
 }
 
 int frg_autotune(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,

@@ -78,6 +78,41 @@ hipError_t launch_refine(const Job &job, hipStream_t stream);
 // zeroes a work-queue counter with a kernel (capturable, unlike a memset node)
 hipError_t launch_reset_counter(unsigned long long *p, hipStream_t stream);
 size_t track_lds_bytes(int block, int precision);
+
+// Allocation-free batch launches: a fixed per-GPU scratch array (frg_tracker.hip) holds the
+// work-queue counter and the descriptions of up to kNoAllocMaxInst instances, staged by a kernel
+// whose arguments carry them (no host copy, so the sequence can be captured in a graph).
+constexpr int kNoAllocMaxInst = 64;
+constexpr size_t kNoAllocScratchBytes = 4096;
+struct NoAllocStage {
+    int n_inst;
+    int64_t inst_first[kNoAllocMaxInst + 1];
+    cx<double> gamma[kNoAllocMaxInst];
+    uint64_t seed[kNoAllocMaxInst], pid0[kNoAllocMaxInst];
+};
+struct NoAllocLayout {
+    unsigned long long *queue;
+    int64_t *inst_first;
+    cx<double> *gamma;
+    uint64_t *seed, *pid0;
+};
+// offsets in the scratch array: queue 0, inst_first 256, gamma 1024, seed 2048, pid0 2560
+__host__ __device__ inline NoAllocLayout noalloc_layout(unsigned char *base) {
+    NoAllocLayout L;
+    L.queue = reinterpret_cast<unsigned long long *>(base);
+    L.inst_first = reinterpret_cast<int64_t *>(base + 256);
+    L.gamma = reinterpret_cast<cx<double> *>(base + 1024);
+    L.seed = reinterpret_cast<uint64_t *>(base + 2048);
+    L.pid0 = reinterpret_cast<uint64_t *>(base + 2560);
+    return L;
+}
+static_assert(256 + 8 * (kNoAllocMaxInst + 1) <= 1024 && 1024 + 16 * kNoAllocMaxInst <= 2048 &&
+                  2048 + 8 * kNoAllocMaxInst <= 2560 && 2560 + 8 * kNoAllocMaxInst <= (int)kNoAllocScratchBytes,
+              "scratch layout");
+static_assert(sizeof(NoAllocStage) <= 4000, "kernel argument size");
+// address of this GPU's scratch array (call with the device current)
+hipError_t noalloc_scratch_address(void **ptr);
+hipError_t launch_stage_instances(const NoAllocStage &a, void *base, hipStream_t stream);
 int track_max_groups_per_cu(int block, int kind, int precision);
 
 }  // namespace frg

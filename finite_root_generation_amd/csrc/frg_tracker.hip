@@ -1337,6 +1337,38 @@ hipError_t launch_debug_solve(const DebugSolveArgs &a, int precision, hipStream_
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------------------------------
+// Fixed scratch of the allocation-free batch entry point: the work-queue counter and up to
+// kNoAllocMaxInst instance descriptions (first path, gamma, seed, first RNG id), one copy per GPU
+// for the life of the process.  Host code reaches it only through the address that
+// noalloc_scratch_address() returns for the current device.
+// ------------------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch_address(void **ptr) {
+    return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
+}
+
+// Writes the instance descriptions (kernel arguments) into the scratch layout and clears the
+// work-queue counter.
+__global__ void stage_instances_kernel(NoAllocStage a, unsigned char *base) {
+    const int i = threadIdx.x;
+    NoAllocLayout L = noalloc_layout(base);
+    if (i < a.n_inst) {
+        L.gamma[i] = a.gamma[i];
+        L.seed[i] = a.seed[i];
+        L.pid0[i] = a.pid0[i];
+    }
+    if (i <= a.n_inst) L.inst_first[i] = a.inst_first[i];
+    if (i == 0) *L.queue = 0ull;
+}
+hipError_t launch_stage_instances(const NoAllocStage &a, void *base, hipStream_t stream) {
+    if (a.n_inst < 1 || a.n_inst > kNoAllocMaxInst) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(stage_instances_kernel, dim3(1), dim3(kNoAllocMaxInst + 1), 0, stream, a,
+                       static_cast<unsigned char *>(base));
+    return hipGetLastError();
+}
+
 __global__ void reset_counter_kernel(unsigned long long *p) {
     if (threadIdx.x == 0) *p = 0ull;
 }

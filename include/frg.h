@@ -37,7 +37,8 @@ enum {
     FRG_EINVAL = -1,      /* bad argument */
     FRG_EHIP = -2,        /* HIP runtime error (device, launch, copy) */
     FRG_ENOMEM = -3,      /* device or host allocation failed */
-    FRG_EUNSUPPORTED = -4 /* option combination not implemented */
+    FRG_EUNSUPPORTED = -4, /* option combination not implemented */
+    FRG_ELIMIT = -5        /* size beyond the fixed limit of an entry point */
 };
 
 enum { FRG_START_UNIFORM = 0, FRG_START_GAUSSIAN = 1 };          /* start draws, frgkinematics.cu:7-8 */
@@ -136,6 +137,23 @@ int frg_track_explicit(frg_ctx *ctx, const frg_c64 *fpm, frg_c64 gamma, const fr
 int frg_track_batch(frg_ctx *ctx, int32_t n_inst, const frg_c64 *fpm, const frg_c64 *gamma, const uint64_t *seeds,
                     const uint64_t *first_path_ids, const int64_t *paths_per_inst, frg_c64 *fpt, int32_t *err,
                     int32_t *path_stats, frg_stats *summary);
+
+/* frg_track_batch on device buffers without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  d_fpm (n_inst x 30), d_fpt (total x 21), d_err
+ * (total) and d_path_stats (total x 8, may be NULL) are device pointers; gamma, seeds,
+ * first_path_ids and paths_per_inst are host arrays of n_inst entries, passed to the GPU as kernel
+ * arguments.  The work-queue counter and the instance descriptions live in a fixed scratch array
+ * of FRG_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends: n_inst is at most
+ * FRG_NOALLOC_MAX_INSTANCES (FRG_ELIMIT beyond).  Launches eagerly on `stream` (NULL: the
+ * context's stream) whatever opts.use_graph says and returns without waiting.  Every call on one
+ * GPU, from any context, shares that array and the call overwrites it: calls that overlap on one
+ * GPU race.  Results are those of frg_track_batch with the same opts. */
+#define FRG_NOALLOC_MAX_INSTANCES 64
+#define FRG_NOALLOC_SCRATCH_BYTES 4096
+int frg_track_batch_noalloc(frg_ctx *ctx, int32_t n_inst, const frg_c64 *d_fpm, const frg_c64 *gamma,
+                            const uint64_t *seeds, const uint64_t *first_path_ids, const int64_t *paths_per_inst,
+                            frg_c64 *d_fpt, int32_t *d_err, int32_t *d_path_stats, void *stream);
 
 /* One measured launch shape of frg_autotune. */
 typedef struct {
