@@ -1,0 +1,160 @@
+"""The batched and the learned workloads on their own inputs.
+
+Config 4 (many instances in one launch): 64 task instances (seeded tasks 500..563, start seeds
+600..663, Gaussian start draws) with path counts log-uniform in [64, 16,384] (the workload's
+[1,024, 262,144] scaled by 1/16 to fit the test budget), FP32/FP64 mixed precision and learned
+step control, launch shape picked by the autotuner, all instances in one persistent launch
+replayed from a hipGraph.  Checked: every finite root solves its own instance's target; the
+finite roots of separate plain FP64 launches (static, classical, one per instance) are found at
+1e-8; the FP64 classical variant of the same batch equals the separate launches bit for bit; and
+the oracle agrees path by path with the plain launches on a sample of instances.
+
+Config 5 (learned step size): the model is trained on 10M step records that the tracker writes on
+the tasks of the 4,096-path and full-size workloads (tasks 0 and 1, start seeds other than the
+measured ones), then run on the 4,096-path workload's own inputs (task 0, start seed 0), in FP64
+and in mixed precision, against the classical controller and the oracle's FP64 roots."""
+import sys
+
+import numpy as np
+import pytest
+
+from finite_root_generation_amd import capi, tasks
+from oracle import oracle as O
+from tests import stephenson_np as S
+from tests.rootcmp import ROOT_TOL, abs_diff, found_anywhere
+from tests.test_gpu_parity import rounding_sensitive
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def model45():
+    """Step model trained on 10M records of the configs' tasks (tasks 0 and 1, start seeds 100+)."""
+    recs = []
+    with capi.Tracker(0, start_mode=capi.START_GAUSSIAN, schedule=capi.SCHED_PERSISTENT) as t:
+        for task, seed in ((0, 100), (1, 101), (0, 102), (1, 103)):
+            fpm, g = tasks.seeded_task(task)
+            say(f"[gpu] step log of task {task}, start seed {seed}, 65536 paths")
+            recs.append(t.collect_step_log(fpm, g, seed=seed, n=65536, cap=4_000_000))
+        rec = np.concatenate(recs)[:10_000_000]
+        say(f"[gpu] training on {rec.shape[0]} records")
+        model, loss = t.train_step_model(rec, steps=4000, batch=4096, lr=2e-3, seed=1)
+    assert rec.shape[0] >= 8_000_000
+    return model
+
+
+def _recall(ref_roots, roots):
+    """Fraction of ref_roots with a root of `roots` within 1e-8 (absolute, every component)."""
+    if len(ref_roots) == 0:
+        return 1.0
+    r = roots[np.argsort(roots[:, 0].real, kind="stable")]
+    return float(np.mean([found_anywhere(r, x) for x in ref_roots]))
+
+
+def test_config4_instances_in_one_graph_launch(model45):
+    rng = np.random.default_rng(4)
+    k_inst = 64
+    counts = np.exp(rng.uniform(np.log(64), np.log(16384), k_inst)).astype(np.int64)
+    fpms, gams = zip(*[tasks.seeded_task(500 + k) for k in range(k_inst)])
+    fpms, gams = np.array(fpms), np.array(gams)
+    seeds = 600 + np.arange(k_inst)
+    pids = np.zeros(k_inst, np.int64)
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    n = int(offs[-1])
+    with capi.Tracker(0, precision=capi.PREC_MIXED, use_graph=1, step_control=capi.STEP_LEARNED,
+                      start_mode=capi.START_GAUSSIAN) as t:
+        t.set_step_model(model45)
+        say(f"[gpu] config 4: autotune on 32768 paths")
+        rows, chosen = t.autotune(fpms[0], gams[0], seed=9, n_sample=32768, first_path_id=1 << 40)
+        say(f"    chosen schedule/block/groups-per-CU {chosen} of {len(rows)} candidates")
+        say(f"[gpu] config 4: {k_inst} instances, {n} paths, one graph launch (and a replay)")
+        fpt, err, st, summ = t.track_batch(fpms, gams, seeds, pids, counts)
+        fpt2, err2, _, summ2 = t.track_batch(fpms, gams, seeds, pids, counts)
+    ok = tasks.finite_mask(fpt, err)
+    # the FP32/FP64 choice is made per wave, and in a persistent launch which paths share a wave
+    # depends on timing, so a mixed-precision replay is not bit-identical: the same roots, path by
+    # path, for nearly every path
+    ok2 = tasks.finite_mask(fpt2, err2)
+    same = (ok & ok2 & (abs_diff(fpt, fpt2) <= ROOT_TOL)).sum()
+    say(f"    replay: same root on the same path for {same} of {ok.sum()} finite roots; finite {ok2.sum()}")
+    assert same >= 0.99 * ok.sum() and abs(int(ok2.sum()) - int(ok.sum())) <= 0.002 * n
+    say(f"    {n / ((summ.kernel_ms + summ.refine_ms) * 1e-3):.0f} paths/s; finite {ok.sum()} of {n}; "
+        f"codes {np.bincount(err, minlength=6).tolist()}; fp32 solves {int(st['fp32_solves'].sum())}")
+    assert ok.mean() > 0.99
+    worst = 0.0
+    for k in range(k_inst):
+        sl = slice(offs[k], offs[k + 1])
+        z = fpt[sl][ok[sl]]
+        if len(z):
+            F = S.fsys(np.concatenate([z, np.ones((len(z), 1))], axis=1), fpms[k][None, :])
+            worst = max(worst, float((np.max(np.abs(F), axis=1) / (1 + np.max(np.abs(z), axis=1)) ** 3).max()))
+    say(f"    worst relative residual of a finite root against its own target: {worst:.1e}")
+    assert worst < 1e-9
+    # the plain way: one static FP64 launch per instance with the classical controller
+    pf, pe = [], []
+    with capi.Tracker(0, start_mode=capi.START_GAUSSIAN) as t:
+        for k in range(k_inst):
+            f, e, _, _ = t.track(fpms[k], gams[k], int(seeds[k]), int(counts[k]), first_path_id=0)
+            pf.append(f)
+            pe.append(e)
+    pf, pe = np.concatenate(pf), np.concatenate(pe)
+    pok = tasks.finite_mask(pf, pe)
+    recalls = [_recall(pf[offs[k]:offs[k + 1]][pok[offs[k]:offs[k + 1]]],
+                       fpt[offs[k]:offs[k + 1]][ok[offs[k]:offs[k + 1]]]) for k in range(k_inst)]
+    rec_all = float(np.average(recalls, weights=[max(1, int(pok[offs[k]:offs[k + 1]].sum())) for k in range(k_inst)]))
+    say(f"    plain finite {pok.sum()}; recall of the plain roots by the batched launch {rec_all:.5f} "
+        f"(worst instance {min(recalls):.4f})")
+    assert rec_all >= 0.99
+    # the FP64 classical variant of the same batch: bit-identical to the separate launches
+    with capi.Tracker(0, schedule=capi.SCHED_PERSISTENT, use_graph=1, start_mode=capi.START_GAUSSIAN) as t:
+        fb, eb, _, _ = t.track_batch(fpms, gams, seeds, pids, counts)
+    assert np.array_equal(eb, pe) and np.array_equal(fb.view(np.float64), pf.view(np.float64), equal_nan=True)
+    # the oracle on the first 32 paths of four instances
+    oo = O.default_opts(start_mode=capi.START_GAUSSIAN)
+    bad, expl = 0, 0
+    for k in (0, 17, 40, 63):
+        m = int(min(32, counts[k]))
+        fo, eo, _, _ = O.homotopy(oo, fpms[k], gams[k], int(seeds[k]), 0, m)
+        fg, eg = pf[offs[k]:offs[k] + m], pe[offs[k]:offs[k] + m]
+        both = tasks.finite_mask(fo, eo) & tasks.finite_mask(fg, eg)
+        agree = (eo == eg) & (~both | (abs_diff(fg, fo) <= ROOT_TOL))
+        for i in np.nonzero(~agree)[0]:
+            bad += 1
+            spt, spm = O.generate_start(oo, int(seeds[k]), int(i))
+            expl += rounding_sensitive(oo, fpms[k], gams[k], spt, spm, int(seeds[k]), int(i))
+    say(f"    oracle sample: {bad} disagreeing paths, {expl} rounding-sensitive in the oracle")
+    assert bad == expl
+
+
+@pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_MIXED], ids=["fp64", "mixed"])
+def test_config5_learned_on_the_4096_path_inputs(model45, precision):
+    fpm, gamma = tasks.seeded_task(0)
+    n = 4096
+    res = {}
+    for name, sc in (("classical", capi.STEP_CLASSIC), ("learned", capi.STEP_LEARNED)):
+        with capi.Tracker(0, precision=precision, step_control=sc, start_mode=capi.START_GAUSSIAN) as t:
+            if sc == capi.STEP_LEARNED:
+                t.set_step_model(model45)
+            say(f"[gpu] config 5 ({'fp64' if precision == 0 else 'mixed'}): {name}, {n} paths")
+            res[name] = t.track(fpm, gamma, seed=0, n=n)
+    fo, eo, _, _ = O.homotopy(O.default_opts(start_mode=capi.START_GAUSSIAN), fpm, gamma, 0, 0, n)
+    oracle_roots = fo[tasks.finite_mask(fo, eo)]
+    out = {}
+    for name, (f, e, st, summ) in res.items():
+        ok = tasks.finite_mask(f, e)
+        out[name] = {"recall": _recall(oracle_roots, f[ok]), "finite": int(ok.sum()),
+                     "rk": float(st["rk_attempts"].mean()),
+                     "solves": float((st["tangent_solves"] + st["newton_its"]).mean()), "ms": summ.kernel_ms}
+        say(f"    {name}: recall of the oracle's FP64 roots {out[name]['recall']:.4f}, finite {out[name]['finite']}, "
+            f"RK attempts/path {out[name]['rk']:.2f}, solves/path {out[name]['solves']:.1f}, kernel {out[name]['ms']:.1f} ms")
+    # FP32 tangents change some paths' end points by themselves (the classical controller's recall
+    # in mixed precision is below its FP64 one), so the learned controller is held to the classical
+    # controller's recall in the same precision
+    assert out["learned"]["recall"] >= min(0.99, out["classical"]["recall"] - 0.005)
+    assert out["learned"]["rk"] < out["classical"]["rk"]
+    assert out["learned"]["solves"] < out["classical"]["solves"]
