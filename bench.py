@@ -247,13 +247,14 @@ def traffic_probe(n):
 def shape_of(capi, o):
     names = {capi.SCHED_STATIC: "static", capi.SCHED_PERSISTENT: "persistent"}
     return {"schedule": names[o.schedule], "block_size": o.block_size or 64, "groups_per_cu": o.groups_per_cu,
-            "lanes_per_path": o.lanes_per_path or 8}
+            "lanes_per_path": o.lanes_per_path or 8, "waves_per_simd": o.waves_per_simd or 1}
 
 
 def tune_table(capi, rows):
     names = {capi.SCHED_STATIC: "static", capi.SCHED_PERSISTENT: "persistent"}
     return [{"schedule": names[r["schedule"]], "block_size": r["block_size"], "groups_per_cu": r["groups_per_cu"],
-             "lanes_per_path": r.get("lanes_per_path", 8), "paths_per_s": round(r["paths_per_s"], 1)} for r in rows]
+             "lanes_per_path": r["lanes_per_path"], "waves_per_simd": r["waves_per_simd"],
+             "paths_per_s": round(r["paths_per_s"], 1)} for r in rows]
 
 
 def batched(capi, tasks, step_model, with_plain=True):
@@ -374,14 +375,23 @@ def main():
                          model=step_model, tune=None if args.no_tune else (9, min(TUNE_SAMPLE, n)))
     ablations = {}
     if not args.no_ablations:
-        for name, kw, mdl in (
-                ("persistent_classical", dict(precision=capi.PREC_FP64, schedule=capi.SCHED_PERSISTENT), None),
-                ("mixed_persistent_classical", dict(precision=capi.PREC_MIXED, schedule=capi.SCHED_PERSISTENT), None),
-                ("mixed_persistent_learned", dict(precision=capi.PREC_MIXED, schedule=capi.SCHED_PERSISTENT,
-                                                  step_control=capi.STEP_LEARNED), step_model)):
+        co = chosen["opts"]
+        variants = [
+            ("persistent_classical", dict(precision=capi.PREC_FP64, schedule=capi.SCHED_PERSISTENT), None),
+            ("mixed_persistent_classical", dict(precision=capi.PREC_MIXED, schedule=capi.SCHED_PERSISTENT), None),
+            ("mixed_persistent_learned", dict(precision=capi.PREC_MIXED, schedule=capi.SCHED_PERSISTENT,
+                                              step_control=capi.STEP_LEARNED), step_model)]
+        if co.lanes_per_path == 16:
+            # the chosen controls with 8 lanes per path: what the 16-lane kernel adds
+            variants.append(("persistent_learned_8_lanes", dict(precision=capi.PREC_FP64,
+                                                                schedule=capi.SCHED_PERSISTENT,
+                                                                step_control=capi.STEP_LEARNED), step_model))
+        for name, kw, mdl in variants:
             o = dict(kw, start_mode=capi.START_GAUSSIAN)
-            if chosen["opts"].lanes_per_path:
-                o["lanes_per_path"] = chosen["opts"].lanes_per_path
+            # the chosen lanes per path where the precision allows it (the 16-lane solve is FP64)
+            if co.lanes_per_path == 16 and kw["precision"] == capi.PREC_FP64 and "8_lanes" not in name:
+                o["lanes_per_path"] = 16
+                o["waves_per_simd"] = co.waves_per_simd
             r = run_variant(capi, torch, name, o, fpm, gamma, n, 1, 0, seed, model=mdl)
             ok_r, ok_p = tasks.finite_mask(r["fpt"], r["err"]), tasks.finite_mask(plain["fpt"], plain["err"])
             ablations[name] = {"value": round(n / (r["ms_per_step"] * 1e-3), 1),

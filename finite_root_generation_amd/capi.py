@@ -67,6 +67,8 @@ class Opts(ct.Structure):
         ("use_graph", ct.c_int32),
         ("lanes_per_path", ct.c_int32),
         ("polish_its", ct.c_int32),
+        ("waves_per_simd", ct.c_int32),
+        ("pad0", ct.c_int32),
     ]
 
 
@@ -74,6 +76,7 @@ class TuneEntry(ct.Structure):
     """Mirror of frg_tune_entry."""
 
     _fields_ = [("schedule", ct.c_int32), ("block_size", ct.c_int32), ("groups_per_cu", ct.c_int32),
+                ("lanes_per_path", ct.c_int32), ("waves_per_simd", ct.c_int32), ("pad0", ct.c_int32),
                 ("kernel_ms", ct.c_double), ("paths_per_s", ct.c_double)]
 
 
@@ -86,7 +89,7 @@ class Stats(ct.Structure):
         ("newton_its", ct.c_int64), ("tangent_solves", ct.c_int64),
         ("kernel_ms", ct.c_double), ("total_ms", ct.c_double),
         ("block_size", ct.c_int32), ("grid_size", ct.c_int32), ("schedule", ct.c_int32), ("precision", ct.c_int32),
-        ("refine_ms", ct.c_double),
+        ("refine_ms", ct.c_double), ("lanes_per_path", ct.c_int32), ("waves_per_simd", ct.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -288,17 +291,17 @@ class Tracker:
 
     def autotune(self, fpm, gamma, seed: int, n_sample: int, first_path_id: int = 0):
         """Measures the candidate launch shapes and keeps the fastest; returns the measurements
-        and the chosen (schedule, block_size, groups_per_cu)."""
+        and the chosen (schedule, block_size, groups_per_cu, lanes_per_path, waves_per_simd)."""
         fpm = _c(fpm, (NP,))
-        table = (TuneEntry * 64)()
+        table = (TuneEntry * 128)()
         k = ct.c_int32()
-        _check(lib().frg_autotune(self._h, _p(fpm), _c64(gamma), seed, first_path_id, n_sample, table, 64,
+        _check(lib().frg_autotune(self._h, _p(fpm), _c64(gamma), seed, first_path_id, n_sample, table, 128,
                                   ct.byref(k)), "frg_autotune")
         rows = [{f: getattr(table[i], f) for f, _ in TuneEntry._fields_} for i in range(k.value)]
         o = Opts()
         _check(lib().frg_get_opts(self._h, ct.byref(o)), "frg_get_opts")
         self.opts = o
-        return rows, (o.schedule, o.block_size, o.groups_per_cu)
+        return rows, (o.schedule, o.block_size, o.groups_per_cu, o.lanes_per_path, o.waves_per_simd)
 
     def collect_step_log(self, fpm, gamma, seed: int, n: int, cap: int, first_path_id: int = 0):
         fpm = _c(fpm, (NP,))

@@ -68,7 +68,7 @@ struct frg_ctx {
     hipGraphExec_t graph_exec = nullptr;
     struct GraphKey {
         frg::Job job;
-        int kind, block, grid;
+        int kind, block, grid, lanes, waves;
     } graph_key;
     bool graph_valid = false;
     // this GPU's fixed scratch array of the allocation-free entry point (device address)
@@ -144,7 +144,10 @@ int check_opts(const frg_opts &o) {
     if (o.block_size != 0 && (o.block_size < 64 || o.block_size > 256 || o.block_size % 64 != 0)) return FRG_EINVAL;
     if (o.groups_per_cu < 0 || o.max_steps < 0 || o.max_ministeps < 0) return FRG_EINVAL;
     if (o.polish_its < 0 || o.polish_its > 8) return FRG_EINVAL;
-    if (o.lanes_per_path != 0 && o.lanes_per_path != 8) return FRG_EUNSUPPORTED;
+    if (o.lanes_per_path != 0 && o.lanes_per_path != 8 && o.lanes_per_path != 16) return FRG_EINVAL;
+    if (o.lanes_per_path == 16 && o.precision != FRG_PREC_FP64) return FRG_EUNSUPPORTED;  // FP64 solve only
+    if (o.waves_per_simd < 0 || o.waves_per_simd > 2) return FRG_EINVAL;
+    if (o.waves_per_simd == 2 && o.lanes_per_path != 16) return FRG_EUNSUPPORTED;  // 16-lane kernel only
     if (!(o.smin > 0.0) || !(o.rk_tol > 0.0)) return FRG_EINVAL;
     return FRG_OK;
 }
@@ -152,6 +155,8 @@ int check_opts(const frg_opts &o) {
 // Default launch shapes: 64-thread blocks (8 paths per block); persistent grids fill every CU
 // to the occupancy the kernel's register use allows.
 int pick_block(const frg_ctx *c) { return c->opts.block_size ? c->opts.block_size : 64; }
+int pick_lanes(const frg_ctx *c) { return c->opts.lanes_per_path ? c->opts.lanes_per_path : 8; }
+int pick_waves(const frg_ctx *c) { return c->opts.waves_per_simd ? c->opts.waves_per_simd : 1; }
 
 // allow_graph = false launches eagerly whatever opts.use_graph says (no graph is created); a job
 // that brings its own work-queue counter (job.queue) uses it instead of the context's.
@@ -161,12 +166,14 @@ int run_job(frg_ctx *c, frg::Job &job, hipStream_t stream, frg_stats *summary, b
     if (job.learned && !c->have_model) return FRG_EINVAL;
     job.model = c->model;
     const int block = pick_block(c);
-    const int gpb = block / 8;
+    const int lanes = pick_lanes(c);
+    const int waves = pick_waves(c);
+    const int gpb = block / lanes;
     int grid;
     int kind = c->opts.schedule == FRG_SCHED_PERSISTENT ? frg::LK_PERSISTENT : frg::LK_STATIC;
     if (kind == frg::LK_PERSISTENT) {
         int per_cu = c->opts.groups_per_cu;
-        if (per_cu == 0) per_cu = frg::track_max_groups_per_cu(block, kind, c->opts.precision);
+        if (per_cu == 0) per_cu = frg::track_max_groups_per_cu(block, kind, c->opts.precision, lanes, waves);
         if (per_cu < gpb) per_cu = gpb;
         grid = std::max(1, (per_cu / gpb) * c->num_cu);
         const int64_t need = (job.n_total + gpb - 1) / gpb;
@@ -189,12 +196,14 @@ int run_job(frg_ctx *c, frg::Job &job, hipStream_t stream, frg_stats *summary, b
         key.kind = kind;
         key.block = block;
         key.grid = grid;
+        key.lanes = lanes;
+        key.waves = waves;
         if (!c->graph_valid || std::memcmp(&key, &c->graph_key, sizeof(key)) != 0) {
             drop_graph(c);
             HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             hipError_t e = hipSuccess;
             if (kind == frg::LK_PERSISTENT) e = frg::launch_reset_counter(job.queue, stream);
-            if (e == hipSuccess) e = frg::launch_track(job, kind, block, grid, stream);
+            if (e == hipSuccess) e = frg::launch_track(job, kind, block, grid, lanes, waves, stream);
             if (e == hipSuccess) e = frg::launch_refine(job, stream);
             hipGraph_t g = nullptr;
             const hipError_t e2 = hipStreamEndCapture(stream, &g);
@@ -216,12 +225,14 @@ int run_job(frg_ctx *c, frg::Job &job, hipStream_t stream, frg_stats *summary, b
     } else {
         if (kind == frg::LK_PERSISTENT) HIPCHK(frg::launch_reset_counter(job.queue, stream));
         HIPCHK(hipEventRecord(c->ev0, stream));
-        HIPCHK(frg::launch_track(job, kind, block, grid, stream));
+        HIPCHK(frg::launch_track(job, kind, block, grid, lanes, waves, stream));
         HIPCHK(hipEventRecord(c->ev_mid, stream));
         HIPCHK(frg::launch_refine(job, stream));
         HIPCHK(hipEventRecord(c->ev1, stream));
     }
     if (summary) {
+        summary->lanes_per_path = lanes;
+        summary->waves_per_simd = waves;
         summary->block_size = block;
         summary->grid_size = grid;
         summary->schedule = c->opts.schedule;
@@ -383,6 +394,7 @@ int frg_default_opts(frg_opts *o) {
     o->step_control = FRG_STEP_CLASSIC;
     o->use_graph = 0;
     o->lanes_per_path = 0;
+    o->waves_per_simd = 0;
     o->polish_its = 2;
     return FRG_OK;
 }
@@ -577,14 +589,23 @@ int frg_autotune(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, u
     job.stats = nullptr;
     job.yfinal = nullptr;
     struct Cand {
-        int schedule, block, gpc;
+        int schedule, block, gpc, lanes, waves;
     };
+    // lanes per path: 8 (8 paths per wave) for every precision, 16 (4 paths per wave) for FP64
+    // with a register budget of 1 or 2 resident waves per SIMD
     std::vector<Cand> cands;
-    for (int block : {64, 128, 256}) cands.push_back({FRG_SCHED_STATIC, block, 0});
-    for (int block : {64, 128, 256}) {
-        const int gpb = block / 8;
-        const int max_gpc = frg::track_max_groups_per_cu(block, frg::LK_PERSISTENT, saved.precision);
-        for (int bpc = std::max(1, max_gpc / gpb); bpc >= 1; --bpc) cands.push_back({FRG_SCHED_PERSISTENT, block, bpc * gpb});
+    const int variants[3][2] = {{8, 1}, {16, 1}, {16, 2}};
+    for (const auto &v : variants) {
+        const int lanes = v[0], waves = v[1];
+        if (lanes == 16 && saved.precision != FRG_PREC_FP64) continue;
+        for (int block : {64, 128, 256}) cands.push_back({FRG_SCHED_STATIC, block, 0, lanes, waves});
+        for (int block : {64, 128, 256}) {
+            const int gpb = block / lanes;
+            const int max_gpc =
+                frg::track_max_groups_per_cu(block, frg::LK_PERSISTENT, saved.precision, lanes, waves);
+            for (int bpc = std::max(1, max_gpc / gpb); bpc >= 1; --bpc)
+                cands.push_back({FRG_SCHED_PERSISTENT, block, bpc * gpb, lanes, waves});
+        }
     }
     int best = -1;
     double best_ms = 0.0;
@@ -594,6 +615,8 @@ int frg_autotune(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, u
         c->opts.schedule = cands[i].schedule;
         c->opts.block_size = cands[i].block;
         c->opts.groups_per_cu = cands[i].gpc;
+        c->opts.lanes_per_path = cands[i].lanes;
+        c->opts.waves_per_simd = cands[i].waves;
         rc = run_job(c, job, c->stream, nullptr);
         if (rc != FRG_OK) {
             c->opts = saved;
@@ -606,6 +629,9 @@ int frg_autotune(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, u
             table[k].schedule = cands[i].schedule;
             table[k].block_size = cands[i].block;
             table[k].groups_per_cu = cands[i].gpc;
+            table[k].lanes_per_path = cands[i].lanes;
+            table[k].waves_per_simd = cands[i].waves;
+            table[k].pad0 = 0;
             table[k].kernel_ms = ms;
             table[k].paths_per_s = ms > 0.f ? n_sample / (ms * 1e-3) : 0.0;
             ++k;
@@ -619,6 +645,8 @@ int frg_autotune(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, u
     c->opts.schedule = cands[best].schedule;
     c->opts.block_size = cands[best].block;
     c->opts.groups_per_cu = cands[best].gpc;
+    c->opts.lanes_per_path = cands[best].lanes;
+    c->opts.waves_per_simd = cands[best].waves;
     drop_graph(c);
     if (n_entries) *n_entries = k;
     return FRG_OK;
@@ -687,6 +715,7 @@ int frg_solve_step(frg_ctx *c, int32_t mode, int32_t precision, int64_t n, const
                    frg_c64 *x) {
     if (!c || n < 0 || mode < 0 || mode > 2) return FRG_EINVAL;
     if (precision != FRG_PREC_FP64 && precision != FRG_PREC_MIXED) return FRG_EINVAL;
+    if (pick_lanes(c) == 16 && precision != FRG_PREC_FP64) return FRG_EUNSUPPORTED;
     if (n == 0) return FRG_OK;
     if (!Y || !u || !spm || !fpm || !x || (mode < 2 && !w) || (mode == 2 && !tgoal)) return FRG_EINVAL;
     BEGIN(c);
@@ -717,7 +746,7 @@ int frg_solve_step(frg_ctx *c, int32_t mode, int32_t precision, int64_t n, const
     a.tgoal = c->dbg_e.as<double>();
     a.gamma = frg::mk(gamma.re, gamma.im);
     a.x = c->dbg_x.as<cx<double>>();
-    HIPCHK(frg::launch_debug_solve(a, precision, c->stream));
+    HIPCHK(frg::launch_debug_solve(a, precision, pick_lanes(c), c->stream));
     HIPCHK(hipMemcpyAsync(x, c->dbg_x.p, v23, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return FRG_OK;

@@ -136,12 +136,15 @@ __host__ __device__ __forceinline__ cx<double> csqrt_ref(cx<double> x) {
 }
 
 // ------------------------------------------------------------------------------------------
-// 8-lane group primitives (DPP).  DPP controls (gfx9): quad_perm [1,0,3,2] = 0xB1,
-// quad_perm [2,3,0,1] = 0x4E, row_half_mirror = 0x141 (lane i <-> 7-i within each 8 lanes).
+// Group primitives (DPP) for groups of W = 8 or 16 lanes inside a wave64 (one path per group).
+// DPP controls (gfx9): quad_perm [1,0,3,2] = 0xB1, quad_perm [2,3,0,1] = 0x4E, row_half_mirror
+// = 0x141 (lane i <-> 7-i within each 8 lanes), row_mirror = 0x140 (lane i <-> 15-i within each
+// 16-lane row).  Each butterfly step combines a lane with its mirror image, so every lane of a
+// group ends with bitwise identical results (x + y = y + x, max is symmetric).
 // ------------------------------------------------------------------------------------------
-// Every control used here reads a lane of the same 8-lane group, so no lane falls outside the
-// row and the old value and bound_ctrl never matter; old = 0 with bound_ctrl set lets the
-// compiler fold the move into the instruction that consumes it (v_max_u32_dpp and the like).
+// Every control used here reads a lane of the same group, so no lane falls outside the row and
+// the old value and bound_ctrl never matter; old = 0 with bound_ctrl set lets the compiler fold
+// the move into the instruction that consumes it (v_max_u32_dpp and the like).
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int v) {
     return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
@@ -161,46 +164,65 @@ template <int CTRL>
 __device__ __forceinline__ double dpp(double x) { return dpp_d<CTRL>(x); }
 template <int CTRL>
 __device__ __forceinline__ float dpp(float x) { return dpp_f<CTRL>(x); }
+template <int CTRL, typename R>
+__device__ __forceinline__ cx<R> dpp(cx<R> z) { return mk(dpp<CTRL>(z.re), dpp<CTRL>(z.im)); }
 
-template <typename R>
+template <int W = 8, typename R>
 __device__ __forceinline__ R grp_sum(R x) {
     x = x + dpp<0xB1>(x);
     x = x + dpp<0x4E>(x);
     x = x + dpp<0x141>(x);
+    if (W == 16) x = x + dpp<0x140>(x);
     return x;
 }
-template <typename R>
+template <int W = 8, typename R>
 __device__ __forceinline__ R grp_max(R x) {
     x = fmax(x, dpp<0xB1>(x));
     x = fmax(x, dpp<0x4E>(x));
     x = fmax(x, dpp<0x141>(x));
+    if (W == 16) x = fmax(x, dpp<0x140>(x));
     return x;
 }
-template <typename R>
-__device__ __forceinline__ cx<R> grp_sum(cx<R> z) { return mk(grp_sum(z.re), grp_sum(z.im)); }
+template <int W = 8, typename R>
+__device__ __forceinline__ cx<R> grp_sum(cx<R> z) { return mk(grp_sum<W>(z.re), grp_sum<W>(z.im)); }
 
-// Value of lane `src` (0..7) of the group, delivered to every lane (sum with zeros elsewhere).
-template <typename R>
-__device__ __forceinline__ R grp_bcast(R x, int role, int src) { return grp_sum(role == src ? x : R(0)); }
-template <typename R>
+// Value of lane `src` of the group, delivered to every lane (sum with zeros elsewhere).
+template <int W = 8, typename R>
+__device__ __forceinline__ R grp_bcast(R x, int role, int src) { return grp_sum<W>(role == src ? x : R(0)); }
+template <int W = 8, typename R>
 __device__ __forceinline__ cx<R> grp_bcast(cx<R> z, int role, int src) {
-    return mk(grp_bcast(z.re, role, src), grp_bcast(z.im, role, src));
+    return mk(grp_bcast<W>(z.re, role, src), grp_bcast<W>(z.im, role, src));
 }
+template <int W = 8>
 __device__ __forceinline__ int grp_bcast_i(int v, int role, int src) {
     int x = (role == src) ? v : 0;
     x = x + dpp_i<0xB1>(x);
     x = x + dpp_i<0x4E>(x);
     x = x + dpp_i<0x141>(x);
+    if (W == 16) x = x + dpp_i<0x140>(x);
     return x;
 }
 
 // Group maximum of an unsigned word (DPP).
+template <int W = 8>
 __device__ __forceinline__ uint32_t grp_max_u32(uint32_t x) {
     x = max(x, (uint32_t)dpp_i<0xB1>((int)x));
     x = max(x, (uint32_t)dpp_i<0x4E>((int)x));
     x = max(x, (uint32_t)dpp_i<0x141>((int)x));
+    if (W == 16) x = max(x, (uint32_t)dpp_i<0x140>((int)x));
     return x;
 }
+
+// Sum and maximum over a lane pair (2i, 2i+1): bitwise identical in both lanes.
+template <typename R>
+__device__ __forceinline__ R pair_sum(R x) { return x + dpp<0xB1>(x); }
+template <typename R>
+__device__ __forceinline__ cx<R> pair_sum(cx<R> z) { return mk(pair_sum(z.re), pair_sum(z.im)); }
+template <typename R>
+__device__ __forceinline__ R pair_max(R x) { return fmax(x, dpp<0xB1>(x)); }
+// The other lane of the pair's value.
+template <typename R>
+__device__ __forceinline__ cx<R> pair_swap(cx<R> z) { return dpp<0xB1>(z); }
 
 // Pivot search key of a row with pivot magnitude |a|^2 = m and id (0..15) in its group: larger
 // m wins, then the smaller id.  Finite m (compared in FP32, to 2^-19 relative) beats NaN, which

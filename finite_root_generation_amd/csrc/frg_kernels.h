@@ -67,17 +67,19 @@ struct DebugStartArgs {
     TrackOpts o;
     cx<double> *spt, *spm;  // n x 21, n x 30 out
 };
-hipError_t launch_debug_solve(const DebugSolveArgs &a, int precision, hipStream_t stream);
+hipError_t launch_debug_solve(const DebugSolveArgs &a, int precision, int lanes, hipStream_t stream);
 hipError_t launch_debug_start(const DebugStartArgs &a, hipStream_t stream);
 
 // host-side launchers (frg_tracker.hip)
 enum LaunchKind : int { LK_STATIC = 0, LK_PERSISTENT = 1 };
-hipError_t launch_track(const Job &job, int kind, int block, int grid, hipStream_t stream);
+// lanes: lanes of a wave64 per path, 8 (FP64 or mixed) or 16 (FP64); waves: register budget of the
+// 16-lane kernel as resident waves per SIMD, 1 (512 VGPRs) or 2 (256); 1 for 8 lanes
+hipError_t launch_track(const Job &job, int kind, int block, int grid, int lanes, int waves, hipStream_t stream);
 // refinement of code-0 roots and cognate canonicalisation of every output row (after the tracker)
 hipError_t launch_refine(const Job &job, hipStream_t stream);
 // zeroes a work-queue counter with a kernel (capturable, unlike a memset node)
 hipError_t launch_reset_counter(unsigned long long *p, hipStream_t stream);
-size_t track_lds_bytes(int block, int precision);
+size_t track_lds_bytes(int block, int lanes);
 
 // Allocation-free batch launches: a fixed per-GPU scratch array (frg_tracker.hip) holds the
 // work-queue counter and the descriptions of up to kNoAllocMaxInst instances, staged by a kernel
@@ -113,6 +115,6 @@ static_assert(sizeof(NoAllocStage) <= 4000, "kernel argument size");
 // address of this GPU's scratch array (call with the device current)
 hipError_t noalloc_scratch_address(void **ptr);
 hipError_t launch_stage_instances(const NoAllocStage &a, void *base, hipStream_t stream);
-int track_max_groups_per_cu(int block, int kind, int precision);
+int track_max_groups_per_cu(int block, int kind, int precision, int lanes, int waves);
 
 }  // namespace frg

@@ -56,16 +56,18 @@ __device__ __forceinline__ void unpack8(uint4 q, float out[8]) {
     }
 }
 
-// Wave-wide evaluation for the groups of one wave that have a pending proposal.  Lane l owns
-// hidden unit l of both layers, so its weights are one row of W1 and one row of W2, read once
-// per call.  Must be called by all 64 lanes with a wave-uniform `lanes` mask whose set bits are
-// lanes 8g of the pending groups g.  Group g has written its normalised features to
-// area0[g * stride + 0..7] (floats in LDS); the same 128-float area then holds that group's
-// first hidden layer (64 FP16 values at float offset +64) and the 64 output-layer terms (at +0).  `gw` is this lane's group
-// in the wave, `role` its lane in the group.  Returns the predicted log step size in the lanes of
-// each pending group.
+// Wave-wide evaluation for the groups of one wave that have a pending proposal; W = 8 or 16
+// lanes per group.  Lane l owns hidden unit l of both layers, so its weights are one row of W1
+// and one row of W2, read once per call.  Must be called by all 64 lanes with a wave-uniform
+// `lanes` mask whose set bits are lanes W*g of the pending groups g.  Group g has written its
+// normalised features to area0[g * stride + 0..7] (floats in LDS); the same 128-float area then
+// holds that group's first hidden layer (64 FP16 values at float offset +64) and the 64
+// output-layer terms (at +0).  `gw` is this lane's group in the wave, `role` its lane in the
+// group.  Returns the predicted log step size in the lanes of each pending group.
+template <int W = 8>
 __device__ __forceinline__ float mlp_eval_wave(const StepModel &M, float *area0, int stride, uint64_t lanes, int gw,
                                                int role) {
+    constexpr int kShift = W == 16 ? 4 : 3;
     const int l = __lane_id();
     const uint4 *w1 = reinterpret_cast<const uint4 *>(M.w + kMlpW1 + kMlpIn * l);
     const uint4 *w2 = reinterpret_cast<const uint4 *>(M.w + kMlpW2 + kMlpH * l);
@@ -81,7 +83,7 @@ __device__ __forceinline__ float mlp_eval_wave(const StepModel &M, float *area0,
     lds_sync();
     // layer 1 for every pending group
     for (uint64_t m = lanes; m; m &= m - 1) {
-        float *area = area0 + (int)(__builtin_ctzll(m) >> 3) * stride;
+        float *area = area0 + (int)(__builtin_ctzll(m) >> kShift) * stride;
         const float4 x0 = reinterpret_cast<const float4 *>(area)[0];
         const float4 x1 = reinterpret_cast<const float4 *>(area)[1];
         float a = b1;
@@ -98,7 +100,7 @@ __device__ __forceinline__ float mlp_eval_wave(const StepModel &M, float *area0,
     lds_sync();
     // layer 2 and the output-layer terms w3[l] tanh(.) for every pending group
     for (uint64_t m = lanes; m; m &= m - 1) {
-        float *area = area0 + (int)(__builtin_ctzll(m) >> 3) * stride;
+        float *area = area0 + (int)(__builtin_ctzll(m) >> kShift) * stride;
         const uint4 *hv = reinterpret_cast<const uint4 *>(area + kMlpH);
         float acc = b2;
 #pragma unroll
@@ -112,11 +114,17 @@ __device__ __forceinline__ float mlp_eval_wave(const StepModel &M, float *area0,
         area[l] = w3 * fast_tanh(acc);
     }
     lds_sync();
-    // each group sums its own 64 terms: 8 per lane, then over the group
-    const float4 *tv = reinterpret_cast<const float4 *>(area0 + gw * stride + 8 * role);
-    const float4 t0 = tv[0], t1 = tv[1];
-    float part = ((t0.x + t0.y) + (t0.z + t0.w)) + ((t1.x + t1.y) + (t1.z + t1.w));
-    part = grp_sum(part);
+    // each group sums its own 64 terms: 64/W per lane, then over the group
+    const float4 *tv = reinterpret_cast<const float4 *>(area0 + gw * stride + (64 / W) * role);
+    const float4 t0 = tv[0];
+    float part;
+    if (W == 8) {
+        const float4 t1 = tv[1];
+        part = ((t0.x + t0.y) + (t0.z + t0.w)) + ((t1.x + t1.y) + (t1.z + t1.w));
+    } else {
+        part = (t0.x + t0.y) + (t0.z + t0.w);
+    }
+    part = grp_sum<W>(part);
     lds_sync();
     const float y = part + __half2float(M.w[kMlpB3]);
     return y * M.ysd + M.ymu;

@@ -30,6 +30,7 @@
 #include "frg_device.h"
 #include "frg_system.h"
 #include "frg_kernels.h"
+#include "frg_solve16.h"
 
 namespace frg {
 
@@ -311,8 +312,8 @@ struct alignas(16) GroupScalars {
     int code, step, ministep, streak, attempt, rej;
 };
 
-// Per-group LDS: the solve scratch plus state that is read once per solve and so lives in LDS
-// rather than in VGPRs (the kernel's occupancy is register-bound, LDS has room).
+// Per-group LDS of the 8-lane layout: the solve scratch plus state that is read once per solve
+// and so lives in LDS rather than in VGPRs (the kernel's occupancy is register-bound).
 struct GroupSlot {
     GroupLds<double> lds;   // also used as GroupLds<float> by FP32 solves
     cx<double> par[8][12];  // per lane: start values s6[0..5] and target - start d6[0..5] of its pose
@@ -324,12 +325,46 @@ static_assert(sizeof(GroupSlot) % 16 == 0 && offsetof(GroupLds<double>, a) % 16 
               "the step model reads float4 vectors from GroupLds::a");
 static_assert(sizeof(GroupLds<double>::a) >= 128 * sizeof(float), "features (8) and hidden layer (64 at +64)");
 
+// Per-group LDS of the 16-lane layout (4,736 bytes: 32 paths on a CU at two waves per SIMD).
+struct GroupSlot16 {
+    GroupLds16 lds;
+    cx<double> par[60];    // the 30 parameters as (start value, target - start) pairs
+    cx<double> vc[16][2];  // per lane: its slots of the current velocity Vc
+    cx<double> u[16][2];   // per lane: its slots of the projective patch u
+    GroupScalars gs;
+};
+static_assert(sizeof(GroupSlot16) % 16 == 0 && offsetof(GroupLds16, a) % 16 == 0,
+              "the step model reads float4 vectors from GroupLds16::a");
+static_assert(sizeof(GroupLds16::a) >= 128 * sizeof(float), "features (8) and hidden layer (64 at +64)");
+
+// Lane layouts.  kW lanes track one path; each lane holds kS slots of every 23-vector
+// Y = [z(21), Z0, t]; t sits in slot kTSlot of lane kTLane, Z0 in slot kZSlot of lane kZLane.
+struct Lanes8 {
+    static constexpr int kW = 8, kS = 3, kTLane = 7, kTSlot = 1, kZLane = 7, kZSlot = 0;
+    using Slot = GroupSlot;
+    // lane r < 7: z[2r], z[2r+1], U_{r+1}; lane 7: Z0, t, -
+    __device__ static int slot_index(int role, int k) {
+        if (role < 7) return k == 2 ? 14 + role : 2 * role + k;
+        return k == 2 ? -1 : 21 + k;
+    }
+};
+struct Lanes16 {
+    static constexpr int kW = 16, kS = 2, kTLane = 15, kTSlot = 0, kZLane = 14, kZSlot = 0;
+    using Slot = GroupSlot16;
+    // lane 2p+h < 14: z[7h+p] and (h = 0) U_{p+1}; lane 14: Z0; lane 15: t
+    __device__ static int slot_index(int role, int k) {
+        if (role < 14) return k == 0 ? 7 * (role & 1) + (role >> 1) : ((role & 1) ? -1 : 14 + (role >> 1));
+        return k == 0 ? 7 + role : -1;
+    }
+};
+
+template <class G>
 struct PathState {
     // vectors (this lane's slots)
-    cx<double> Yc[3], K[8][3];  // K[0] doubles as YN, K[1] as Yn after an RK attempt
-    cx<double> *u;              // this lane's row of GroupSlot::u
-    cx<double> *par;                 // this lane's row of GroupSlot::par (s6, d6)
-    cx<double> *vc;                  // this lane's row of GroupSlot::vc
+    cx<double> Yc[G::kS], K[8][G::kS];  // K[0] doubles as YN, K[1] as Yn after an RK attempt
+    cx<double> *u;                       // this lane's row of the group's u
+    cx<double> *par;                     // 8 lanes: this lane's pose parameters; 16: the group's 30
+    cx<double> *vc;                      // this lane's row of the group's vc
     GroupScalars *gs;
     // scalars (identical in all lanes of the group)
     double s, residual;
@@ -343,46 +378,52 @@ struct PathState {
 };
 
 // Patch draw (htpytrack.cu:55-58,147-150): Y index i uses draws base + 2i, base + 2i + 1.
-__device__ __forceinline__ void draw_vec(int role, const PathState &P, uint64_t base, double lo, double hi,
-                                         cx<double> out[3]) {
+template <class G>
+__device__ __forceinline__ void draw_vec(int role, const PathState<G> &P, uint64_t base, double lo, double hi,
+                                         cx<double> out[G::kS]) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const int yi = slot_index(role, k);
+    for (int k = 0; k < G::kS; ++k) {
+        const int yi = G::slot_index(role, k);
         double a, b;
         uniform_pair(base / 2 + (uint64_t)(yi < 0 ? 0 : yi), P.gs->pid, P.gs->seed, a, b);
         out[k] = sel(yi < 0, mk(0.0, 0.0), mk((hi - lo) * a + lo, (hi - lo) * b + lo));
     }
 }
 
+template <class G>
+__device__ __forceinline__ bool is_t_slot(int role, int k) { return role == G::kTLane && k == G::kTSlot; }
+
 // SwitchPatch (htpysupport.cu:9-34): Y' = (1 - u_t t)/(sum_{i<22} u_i Y_i) * Y, t unchanged.
-__device__ __forceinline__ void switch_patch(int role, const cx<double> u[3], cx<double> Y[3]) {
+template <class G>
+__device__ __forceinline__ void switch_patch(int role, const cx<double> u[G::kS], cx<double> Y[G::kS]) {
     cx<double> den = mk(0.0, 0.0), num = mk(0.0, 0.0);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const bool is_t = (role == 7 && k == 1);
+    for (int k = 0; k < G::kS; ++k) {
+        const bool is_t = is_t_slot<G>(role, k);
         den = sel(is_t, den, fma(den, u[k], Y[k]));
         num = sel(is_t, u[k] * Y[k], num);
     }
-    den = grp_sum(den);
-    num = grp_sum(num);
+    den = grp_sum<G::kW>(den);
+    num = grp_sum<G::kW>(num);
     const cx<double> sc = cdiv(mk(1.0 - num.re, -num.im), den);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const bool is_t = (role == 7 && k == 1);
-        Y[k] = sel(is_t, Y[k], sc * Y[k]);
-    }
+    for (int k = 0; k < G::kS; ++k) Y[k] = sel(is_t_slot<G>(role, k), Y[k], sc * Y[k]);
 }
 
-__device__ __forceinline__ double grp_norm(const cx<double> v[3]) {
-    double a = abs2(v[0]) + abs2(v[1]) + abs2(v[2]);
-    return sqrt(grp_sum(a));
+template <class G>
+__device__ __forceinline__ double grp_norm(const cx<double> v[G::kS]) {
+    double a = abs2(v[0]);
+#pragma unroll
+    for (int k = 1; k < G::kS; ++k) a = a + abs2(v[k]);
+    return sqrt(grp_sum<G::kW>(a));
 }
 
 // Tangent normalisation (htpysupport.cu:285-296).  Arclength: x * e^{-i arg x_t} / |x|, written
 // as x * conj(x_t)/(|x_t| |x|); t-step: x / x_t.  Then Im x_t = 0.
-__device__ __forceinline__ void normalize_tangent(int role, cx<double> x[3], bool tstep) {
-    const double nrm = grp_norm(x);
-    const cx<double> xt = grp_bcast(x[1], role, 7);
+template <class G>
+__device__ __forceinline__ void normalize_tangent(int role, cx<double> x[G::kS], bool tstep) {
+    const double nrm = grp_norm<G>(x);
+    const cx<double> xt = grp_bcast<G::kW>(x[G::kTSlot], role, G::kTLane);
     cx<double> sc;
     if (!tstep) {
         const double a = sqrt(abs2(xt));
@@ -391,17 +432,21 @@ __device__ __forceinline__ void normalize_tangent(int role, cx<double> x[3], boo
         sc = inv(xt);
     }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) x[k] = sc * x[k];
-    if (role == 7) x[1].im = 0.0;
+    for (int k = 0; k < G::kS; ++k) x[k] = sc * x[k];
+    if (role == G::kTLane) x[G::kTSlot].im = 0.0;
 }
-// t (real part of Y[22], held by lane 7 slot 1), delivered to the whole group
-__device__ __forceinline__ double grp_t(int role, const cx<double> Y[3]) { return grp_bcast(Y[1].re, role, 7); }
+// t (real part of Y[22]), delivered to the whole group
+template <class G>
+__device__ __forceinline__ double grp_t(int role, const cx<double> Y[G::kS]) {
+    return grp_bcast<G::kW>(Y[G::kTSlot].re, role, G::kTLane);
+}
 
 // ------------------------------------------------------------------------------------------
-// Start system: GenerateStart (frgkinematics.cu:13-136) for pose j = role+1 (lane 7 repeats
-// pose 7 and keeps only Z0 = 1, t = 0).
+// Start system: GenerateStart (frgkinematics.cu:13-136) for pose j (the lane's pose; a lane
+// without a pose repeats pose 7).
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ cx<double> start_value(const TrackOpts &o, uint64_t blk, const PathState &P) {
+template <class G>
+__device__ __forceinline__ cx<double> start_value(const TrackOpts &o, uint64_t blk, const PathState<G> &P) {
     double a, b;
     uniform_pair(blk, P.gs->pid, P.gs->seed, a, b);
     if (o.start_mode == 1) {
@@ -413,26 +458,32 @@ __device__ __forceinline__ cx<double> start_value(const TrackOpts &o, uint64_t b
     return mk((o.start_hi - o.start_lo) * a + o.start_lo, (o.start_hi - o.start_lo) * b + o.start_lo);
 }
 
-__device__ __forceinline__ void generate_start(int role, const TrackOpts &o, PathState &P) {
-    const int j = role < 7 ? role + 1 : 7;
-    const cx<double> one = mk(1.0, 0.0), zero = mk(0.0, 0.0);
-    cx<double> z[14];
+// The start point's linkage z[0..13] and pose j's rotation U and parameters
+// (P0, P_j, Pc0, Pc_j, Q_j, Qc_j).
+struct StartPose {
+    cx<double> z[14], U, q[6];
+};
+
+template <class G>
+__device__ __forceinline__ void start_pose(const TrackOpts &o, const PathState<G> &P, int j, StartPose &sp) {
+    const cx<double> zero = mk(0.0, 0.0);
+    cx<double> *z = sp.z;
 #pragma unroll
     for (int i = 0; i < 14; ++i) z[i] = start_value(o, (uint64_t)i, P);
     const cx<double> P0 = start_value(o, 14, P), Pc0 = start_value(o, 15, P);
     const cx<double> R = start_value(o, 15 + (uint64_t)j, P), iR = inv(R);
     double pmA, pmB;
     uniform_pair(22 + (uint64_t)j, P.gs->pid, P.gs->seed, pmA, pmB);
-    const cx<double> A = z[0], B = z[1], C = z[2], D = z[3], F = z[4], G = z[5], H = z[6];
+    const cx<double> A = z[0], B = z[1], C = z[2], D = z[3], F = z[4], G_ = z[5], H = z[6];
     const cx<double> Ac = z[7], Bc = z[8], Cc = z[9], Dc = z[10], Fc = z[11], Gc = z[12], Hc = z[13];
-    cx<double> Aq = (G - D) * (iR * (Hc - Gc) - (Hc - Fc));
-    cx<double> Acq = (Gc - Dc) * (R * (H - G) - (H - F));
-    cx<double> Bq = ((zero - R * ((H - G) * (Hc - Fc))) - iR * ((Hc - Gc) * (H - F))) +
-                    ((G - D) * (Gc - Dc) + ((H - G) * (Hc - Gc) + ((H - F) * (Hc - Fc) - (F - D) * (Fc - Dc))));
+    cx<double> Aq = (G_ - D) * (iR * (Hc - Gc) - (Hc - Fc));
+    cx<double> Acq = (Gc - Dc) * (R * (H - G_) - (H - F));
+    cx<double> Bq = ((zero - R * ((H - G_) * (Hc - Fc))) - iR * ((Hc - Gc) * (H - F))) +
+                    ((G_ - D) * (Gc - Dc) + ((H - G_) * (Hc - Gc) + ((H - F) * (Hc - Fc) - (F - D) * (Fc - Dc))));
     double pm = pmA < 0.5 ? -1.0 : 1.0;
     const cx<double> TUc = inv(mk(2.0, 0.0) * Aq) *
                            ((zero - Bq) + mk(pm, 0.0) * csqrt_ref(Bq * Bq - mk(4.0, 0.0) * (Aq * Acq)));
-    const cx<double> SUc = inv(F - D) * (TUc * (G - D) + (R * (H - G) - (H - F)));
+    const cx<double> SUc = inv(F - D) * (TUc * (G_ - D) + (R * (H - G_) - (H - F)));
     const cx<double> iSUc = inv(SUc);
     const cx<double> E = (R * (H - C) - SUc * (F - B)) - (H - F);
     const cx<double> Ec = (iR * (Hc - Cc) - iSUc * (Fc - Bc)) - (Hc - Fc);
@@ -445,29 +496,71 @@ __device__ __forceinline__ void generate_start(int role, const TrackOpts &o, Pat
     const cx<double> QUc = mk(-1.0, 0.0) * inv(C - A) * (inv(U) * (A - B) + E);
     const cx<double> Q = QUc * U, Qc = inv(Q);
     const cx<double> S = SUc * U;
-    const cx<double> Pj = B + (S * (F - B) + U * (P0 - F));
-    const cx<double> Pcj = Bc + (inv(S) * (Fc - Bc) + inv(U) * (Pc0 - Fc));
-    P.par[0] = P0;
-    P.par[1] = Pj;
-    P.par[2] = Pc0;
-    P.par[3] = Pcj;
-    P.par[4] = Q;
-    P.par[5] = Qc;
-    if (role < 7) {
-        cx<double> a0 = z[0], a1 = z[1];
+    sp.U = U;
+    sp.q[0] = P0;
+    sp.q[1] = B + (S * (F - B) + U * (P0 - F));
+    sp.q[2] = Pc0;
+    sp.q[3] = Bc + (inv(S) * (Fc - Bc) + inv(U) * (Pc0 - Fc));
+    sp.q[4] = Q;
+    sp.q[5] = Qc;
+}
+
+// Pose of a lane: 8 lanes: lane r < 7 -> pose r+1; 16 lanes: lane 2p+h < 14 -> pose p+1; the
+// patch lanes repeat pose 7.
+template <class G>
+__device__ __forceinline__ int lane_pose(int role) {
+    if (G::kW == 8) return role < 7 ? role + 1 : 7;
+    return role < 14 ? (role >> 1) + 1 : 7;
+}
+
+// Start values of this lane's slots and parameters: GenerateStart (frgkinematics.cu:13-136), or
+// the caller's explicit start system (spt, spm rows).  8 lanes: the lane's six pose parameters
+// go to par[0..5]; 16 lanes: the group's 30 parameters to par[2i] (two per lane).
+template <class G>
+__device__ __forceinline__ void start_values(int role, const Job &J, PathState<G> &P, int64_t path) {
+    const int j = lane_pose<G>(role);
+    const int pidx[6] = {0, j, 8, 8 + j, 15 + j, 22 + j};  // P0, P_j, Pc0, Pc_j, Q_j, Qc_j
+    cx<double> y[G::kS];
+    if (J.spt) {
+        const cx<double> *zs = J.spt + path * 21;
+        const cx<double> *qs = J.spm + path * 30;
 #pragma unroll
-        for (int i = 1; i < 7; ++i) {
-            a0 = sel(role == i, z[2 * i], a0);
-            a1 = sel(role == i, z[2 * i + 1], a1);
+        for (int k = 0; k < G::kS; ++k) {
+            const int yi = G::slot_index(role, k);
+            y[k] = yi < 0 ? mk(0.0, 0.0) : (yi < 21 ? zs[yi] : mk(yi == 21 ? 1.0 : 0.0, 0.0));
         }
-        P.Yc[0] = a0;
-        P.Yc[1] = a1;
-        P.Yc[2] = U;
+        if (G::kW == 8) {
+#pragma unroll
+            for (int p = 0; p < 6; ++p) P.par[p] = qs[pidx[p]];
+        } else {
+            P.par[2 * role] = qs[role];
+            if (role + 16 < 30) P.par[2 * (role + 16)] = qs[role + 16];
+        }
     } else {
-        P.Yc[0] = one;
-        P.Yc[1] = zero;
-        P.Yc[2] = zero;
+        StartPose sp;
+        start_pose(J.o, P, j, sp);
+#pragma unroll
+        for (int k = 0; k < G::kS; ++k) {
+            const int yi = G::slot_index(role, k);
+            cx<double> v = mk(yi == 21 ? 1.0 : 0.0, 0.0);
+#pragma unroll
+            for (int i = 0; i < 14; ++i) v = sel(yi == i, sp.z[i], v);
+            y[k] = sel(yi >= 14 && yi < 21, sp.U, v);
+        }
+        if (G::kW == 8) {
+#pragma unroll
+            for (int p = 0; p < 6; ++p) P.par[p] = sp.q[p];
+        } else if (role < 14) {
+            const int h = role & 1;
+            // the even lane of pose j stores P_j and Q_j, the odd one Pc_j and Qc_j; lanes 0 and
+            // 1 also P0 and Pc0
+            P.par[2 * pidx[1 + 2 * h]] = sp.q[1 + 2 * h];
+            P.par[2 * pidx[4 + h]] = sp.q[4 + h];
+            if (role < 2) P.par[2 * pidx[2 * h]] = sp.q[2 * h];
+        }
     }
+#pragma unroll
+    for (int k = 0; k < G::kS; ++k) P.Yc[k] = y[k];
 }
 
 // Task instance of a global path index: the last instance whose first path is <= path.
@@ -485,7 +578,8 @@ __device__ __forceinline__ int instance_of(const Job &J, int64_t path) {
 // ------------------------------------------------------------------------------------------
 // Path set-up (Homotopy kernel start + TrackPath initialisation, htpytrack.cu:45-77)
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, int64_t path) {
+template <class G>
+__device__ __forceinline__ void init_path(int role, const Job &J, PathState<G> &P, int64_t path) {
     const TrackOpts &o = J.o;
     P.gs->path = path;
     const int inst = instance_of(J, path);
@@ -494,36 +588,27 @@ __device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, 
     P.gs->seed = J.seed[inst];
     P.gs->pid = J.pid0[inst] + (uint64_t)local;
     P.gs->gamma = J.gamma[inst];
-    const int j = role < 7 ? role + 1 : 7;
     const cx<double> *f = J.fpm + (int64_t)inst * 30;
-    const int pidx[6] = {0, j, 8, 8 + j, 15 + j, 22 + j};  // P0, P_j, Pc0, Pc_j, Q_j, Qc_j
-    if (J.spt) {
-        const cx<double> *zs = J.spt + path * 21;
-        const cx<double> *qs = J.spm + path * 30;
+    start_values(role, J, P, path);
+    if (G::kW == 8) {
+        const int j = lane_pose<G>(role);
+        const int pidx[6] = {0, j, 8, 8 + j, 15 + j, 22 + j};
 #pragma unroll
-        for (int p = 0; p < 6; ++p) P.par[p] = qs[pidx[p]];
-        if (role < 7) {
-            P.Yc[0] = zs[2 * role];
-            P.Yc[1] = zs[2 * role + 1];
-            P.Yc[2] = zs[14 + role];
-        } else {
-            P.Yc[0] = mk(1.0, 0.0);
-            P.Yc[1] = mk(0.0, 0.0);
-            P.Yc[2] = mk(0.0, 0.0);
-        }
+        for (int p = 0; p < 6; ++p) P.par[6 + p] = f[pidx[p]] - P.par[p];
     } else {
-        generate_start(role, o, P);
+        lds_sync();
+        P.par[2 * role + 1] = f[role] - P.par[2 * role];
+        if (role + 16 < 30) P.par[2 * (role + 16) + 1] = f[role + 16] - P.par[2 * (role + 16)];
+        lds_sync();
     }
-#pragma unroll
-    for (int p = 0; p < 6; ++p) P.par[6 + p] = f[pidx[p]] - P.par[p];
     // TrackPath reads the same stream from draw 0 (frg.cu:54,57)
     draw_vec(role, P, 0, o.patch_lo, o.patch_hi, P.u);
-    switch_patch(role, P.u, P.Yc);
+    switch_patch<G>(role, P.u, P.Yc);
     {
-        cx<double> v[3];
+        cx<double> v[G::kS];
         draw_vec(role, P, 46, o.patch_lo, o.patch_hi, v);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) P.vc[k] = v[k];
+        for (int k = 0; k < G::kS; ++k) P.vc[k] = v[k];
     }
     P.gs->draw = 92;
     P.phase = PH_INIT;
@@ -549,7 +634,7 @@ __device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, 
 #pragma unroll
     for (int m = 0; m < 8; ++m)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) P.K[m][k] = mk(0.0, 0.0);
+        for (int k = 0; k < G::kS; ++k) P.K[m][k] = mk(0.0, 0.0);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -561,19 +646,21 @@ __device__ __forceinline__ void init_path(int role, const Job &J, PathState &P, 
 // rungekutta.cu:43-104).  Clearing those slots when an attempt starts keeps a stale non-finite
 // value (the YN of a failed corrector, a stage of a rejected attempt) out of the sum: 0 * NaN is
 // NaN.
-template <int kFirst>
-__device__ __forceinline__ void clear_stages(PathState &P) {
+template <int kFirst, class G>
+__device__ __forceinline__ void clear_stages(PathState<G> &P) {
 #pragma unroll
     for (int m = kFirst; m < 8; ++m)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) P.K[m][k] = mk(0.0, 0.0);
+        for (int k = 0; k < G::kS; ++k) P.K[m][k] = mk(0.0, 0.0);
 }
-__device__ __forceinline__ void start_rk(PathState &P) {
+template <class G>
+__device__ __forceinline__ void start_rk(PathState<G> &P) {
     P.phase = PH_RK;
     P.stage = 0;
     clear_stages<0>(P);
 }
-__device__ __forceinline__ void main_loop_top(const TrackOpts &o, PathState &P) {
+template <class G>
+__device__ __forceinline__ void main_loop_top(const TrackOpts &o, PathState<G> &P) {
     if (P.gs->step == o.max_steps) {  // :85
         P.gs->code = 1;
         P.done = true;
@@ -581,27 +668,31 @@ __device__ __forceinline__ void main_loop_top(const TrackOpts &o, PathState &P) 
     }
     start_rk(P);
 }
-__device__ __forceinline__ void start_newton(const TrackOpts &o, PathState &P, double tgoal, bool sharp) {
+template <class G>
+__device__ __forceinline__ void start_newton(const TrackOpts &o, PathState<G> &P, double tgoal, bool sharp) {
     // YN = Yn (K[1] -> K[0])
 #pragma unroll
-    for (int k = 0; k < 3; ++k) P.K[0][k] = P.K[1][k];
+    for (int k = 0; k < G::kS; ++k) P.K[0][k] = P.K[1][k];
     P.its = 0;
     P.residual = 100.0;
     P.tgoal = tgoal;
     P.phase = sharp ? PH_SHARP : PH_NEWTON;
 }
-__device__ __forceinline__ void redraw_patch(int role, const TrackOpts &o, PathState &P) {
+template <class G>
+__device__ __forceinline__ void redraw_patch(int role, const TrackOpts &o, PathState<G> &P) {
     draw_vec(role, P, P.gs->draw, o.patch_lo, o.patch_hi, P.u);
     P.gs->draw += 46;
     P.gs->st_redraws++;
-    switch_patch(role, P.u, P.K[1]);  // only Yn is moved to the new patch (:151, :210)
+    switch_patch<G>(role, P.u, P.K[1]);  // only Yn is moved to the new patch (:151, :210)
 }
-__device__ __forceinline__ void start_sharpen(const TrackOpts &o, PathState &P) {
+template <class G>
+__device__ __forceinline__ void start_sharpen(const TrackOpts &o, PathState<G> &P) {
     P.gs->attempt = 0;
     start_newton(o, P, 1.0, true);
 }
 // corrector entry (:133-135)
-__device__ __forceinline__ void correct(int role, const TrackOpts &o, PathState &P, double tn) {
+template <class G>
+__device__ __forceinline__ void correct(int role, const TrackOpts &o, PathState<G> &P, double tn) {
     if (fabs(1. - tn) < o.sharp_trigger) {
         start_sharpen(o, P);
         return;
@@ -610,12 +701,13 @@ __device__ __forceinline__ void correct(int role, const TrackOpts &o, PathState 
     start_newton(o, P, tn, false);
 }
 // end of an RK attempt (:94-123)
-__device__ __forceinline__ void finish_rk(int role, const TrackOpts &o, PathState &P) {
+template <class G>
+__device__ __forceinline__ void finish_rk(int role, const TrackOpts &o, PathState<G> &P) {
     P.gs->st_rk++;
     const double s = P.s;
-    cx<double> lte[3], yn[3];
+    cx<double> lte[G::kS], yn[G::kS];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < G::kS; ++k) {
         cx<double> e = mk(0.0, 0.0), b = mk(0.0, 0.0);
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
@@ -626,13 +718,13 @@ __device__ __forceinline__ void finish_rk(int role, const TrackOpts &o, PathStat
         lte[k] = e;
         yn[k] = P.Yc[k] + b;
     }
-    const double err = grp_norm(lte);
+    const double err = grp_norm<G>(lte);
     double scale = err > 10e-17 ? 0.8 * pow(o.rk_tol / err, 1. / 6.) : 5.0;
     if (scale > 5) scale = 5;
     P.gs->err = err;
     P.gs->scale = scale;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) P.K[1][k] = yn[k];  // Yn (k2 is not used after the attempt)
+    for (int k = 0; k < G::kS; ++k) P.K[1][k] = yn[k];  // Yn (k2 is not used after the attempt)
     if (P.gs->ministep++ == o.max_ministeps) {  // :95
         P.gs->code = 2;
         P.done = true;
@@ -655,25 +747,25 @@ __device__ __forceinline__ void finish_rk(int role, const TrackOpts &o, PathStat
         clear_stages<1>(P);
         return;
     }
-    const double tn = grp_t(role, P.K[1]);
+    const double tn = grp_t<G>(role, P.K[1]);
     if (tn <= 1. - o.nudge_trigger || fabs(1. - tn) < o.sharp_trigger) {  // :115-118
         correct(role, o, P, tn);
         return;
     }
     P.tstep = true;  // land on t = 1 (:121-123)
-    P.s = 1 - grp_t(role, P.Yc);
+    P.s = 1 - grp_t<G>(role, P.Yc);
     start_rk(P);
 }
-// end of a corrector Newton call (:140-164)
 // Features of the step proposal made at an accepted step (see frg_mlp.h), identical in every
 // lane of the group.
-__device__ __forceinline__ void step_features(int role, const PathState &P, float f[kMlpIn]) {
-    const double t = grp_t(role, P.Yc);
-    const double ynorm = grp_norm(P.Yc);
-    cx<double> v[3];
+template <class G>
+__device__ __forceinline__ void step_features(int role, const PathState<G> &P, float f[kMlpIn]) {
+    const double t = grp_t<G>(role, P.Yc);
+    const double ynorm = grp_norm<G>(P.Yc);
+    cx<double> v[G::kS];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) v[k] = P.vc[k];
-    const double vt = fabs(grp_bcast(v[1].re, role, 7));
+    for (int k = 0; k < G::kS; ++k) v[k] = P.vc[k];
+    const double vt = fabs(grp_bcast<G::kW>(v[G::kTSlot].re, role, G::kTLane));
     auto lg = [](double x) { return __logf(fminf(fmaxf((float)x, 1e-30f), 1e30f)); };
     f[0] = (float)t;
     f[1] = lg(P.s);
@@ -686,7 +778,8 @@ __device__ __forceinline__ void step_features(int role, const PathState &P, floa
 }
 
 // Step size bounds and the next RK attempt (htpytrack.cu:178-183, then the loop top).
-__device__ __forceinline__ void set_step(const TrackOpts &o, PathState &P, double s_next) {
+template <class G>
+__device__ __forceinline__ void set_step(const TrackOpts &o, PathState<G> &P, double s_next) {
     P.s = s_next;
     if (P.s > o.smax) P.s = o.smax;
     else if (P.s < o.smin) P.s = o.smin * 0.9;
@@ -697,12 +790,14 @@ __device__ __forceinline__ void set_step(const TrackOpts &o, PathState &P, doubl
 // s *= scale (scale = 0.8 (tol/err)^(1/6), capped at 5, at most 1 within cautionSteps of a
 // rejection), or -- learned control, arclength steps -- from the model, with the same cap and
 // caution rule and the classical value as the fallback for a non-finite proposal.  A learned
-// proposal is left pending (features in the group's LDS area `xarea`): the wave evaluates the
-// model for all its pending groups at once and finish_proposal completes the step.
-__device__ __forceinline__ void accept_step(int role, const Job &J, PathState &P, float *xarea) {
+// proposal is left pending (features in the group's LDS area `xarea`, one per lane 0..7): the
+// wave evaluates the model for all its pending groups at once and finish_proposal completes the
+// step.
+template <class G>
+__device__ __forceinline__ void accept_step(int role, const Job &J, PathState<G> &P, float *xarea) {
     const TrackOpts &o = J.o;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < G::kS; ++k) {
         P.Yc[k] = P.K[0][k];
         P.vc[k] = P.K[7][k];
     }
@@ -739,7 +834,7 @@ __device__ __forceinline__ void accept_step(int role, const Job &J, PathState &P
             sd = role == q ? J.model.xsd[q] : sd;
         }
         lds_sync();
-        xarea[role] = (fr - mu) / sd;
+        if (role < kMlpIn) xarea[role] = (fr - mu) / sd;
         P.s_classic = s_classic;
         P.pending = true;
         return;
@@ -748,7 +843,8 @@ __device__ __forceinline__ void accept_step(int role, const Job &J, PathState &P
 }
 
 // Completes a pending learned proposal with the model output y (log step size).
-__device__ __forceinline__ void finish_proposal(const TrackOpts &o, PathState &P, float y) {
+template <class G>
+__device__ __forceinline__ void finish_proposal(const TrackOpts &o, PathState<G> &P, float y) {
     double s_next = P.s_classic;
     // safety factor on the model's step: 0.6 minimised RK attempts per path in a sweep over
     // 0.35..1.15 (2^18 paths of a seeded task): fewer rejections outweigh the shorter steps
@@ -762,7 +858,9 @@ __device__ __forceinline__ void finish_proposal(const TrackOpts &o, PathState &P
     set_step(o, P, s_next);
 }
 
-__device__ __forceinline__ void finish_newton(int role, const Job &J, PathState &P, float *xarea) {
+// end of a corrector Newton call (:140-164)
+template <class G>
+__device__ __forceinline__ void finish_newton(int role, const Job &J, PathState<G> &P, float *xarea) {
     const TrackOpts &o = J.o;
     if (P.gs->ministep++ == o.max_ministeps) {
         P.gs->code = 2;
@@ -777,7 +875,7 @@ __device__ __forceinline__ void finish_newton(int role, const Job &J, PathState 
     redraw_patch(role, o, P);
     P.gs->attempt++;
     if (P.gs->attempt < o.newt_attempts) {
-        start_newton(o, P, grp_t(role, P.K[1]), false);
+        start_newton(o, P, grp_t<G>(role, P.K[1]), false);
         return;
     }
     if (P.s <= o.smin) {  // :159
@@ -790,22 +888,22 @@ __device__ __forceinline__ void finish_newton(int role, const Job &J, PathState 
     P.gs->streak = 0;
     start_rk(P);
 }
+
 // The homogeneous end point (before dehomogenisation) for callers that asked for it.
-__device__ __forceinline__ void store_yfinal(int role, const Job &J, int64_t path, const cx<double> Y[3]) {
+template <class G>
+__device__ __forceinline__ void store_yfinal(int role, const Job &J, int64_t path, const cx<double> Y[G::kS]) {
     if (!J.yfinal) return;
     cx<double> *yo = J.yfinal + path * 23;
-    if (role < 7) {
-        yo[2 * role] = Y[0];
-        yo[2 * role + 1] = Y[1];
-        yo[14 + role] = Y[2];
-    } else {
-        yo[21] = Y[0];
-        yo[22] = Y[1];
+#pragma unroll
+    for (int k = 0; k < G::kS; ++k) {
+        const int yi = G::slot_index(role, k);
+        if (yi >= 0) yo[yi] = Y[k];
     }
 }
 
 // end of a sharpening Newton call (:198-216)
-__device__ __forceinline__ void finish_sharp(int role, const Job &J, PathState &P) {
+template <class G>
+__device__ __forceinline__ void finish_sharp(int role, const Job &J, PathState<G> &P) {
     const TrackOpts &o = J.o;
     if (P.gs->ministep++ == o.max_ministeps) {
         P.gs->code = 2;
@@ -831,21 +929,22 @@ __device__ __forceinline__ void finish_sharp(int role, const Job &J, PathState &
 // Output of the tracker: dehomogenise (frg.cu:60) and store the 21 values with the code and the
 // counters.  The refinement kernel below then polishes code-0 roots and applies CognateSolns.
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ void finish_path(int role, const Job &J, PathState &P) {
+template <class G>
+__device__ __forceinline__ void finish_path(int role, const Job &J, PathState<G> &P) {
     // output point: Yc for codes 1-4 and code 2 before sharpening, the sharpened point otherwise
     const bool from_sharp = (P.phase == PH_SHARP);
-    cx<double> Y[3];
+    cx<double> Y[G::kS];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) Y[k] = sel(from_sharp, P.K[0][k], P.Yc[k]);
+    for (int k = 0; k < G::kS; ++k) Y[k] = sel(from_sharp, P.K[0][k], P.Yc[k]);
     const int64_t path = P.gs->path;
-    store_yfinal(role, J, path, Y);
+    store_yfinal<G>(role, J, path, Y);
     // dehomogenise: divide z and Z0 by Z0
-    const cx<double> iZ0 = inv(grp_bcast(Y[0], role, 7));
-    if (role < 7) {
-        cx<double> *out = J.fpt + path * 21;
-        out[2 * role] = Y[0] * iZ0;
-        out[2 * role + 1] = Y[1] * iZ0;
-        out[14 + role] = Y[2] * iZ0;
+    const cx<double> iZ0 = inv(grp_bcast<G::kW>(Y[G::kZSlot], role, G::kZLane));
+    cx<double> *out = J.fpt + path * 21;
+#pragma unroll
+    for (int k = 0; k < G::kS; ++k) {
+        const int yi = G::slot_index(role, k);
+        if (yi >= 0 && yi < 21) out[yi] = Y[k] * iZ0;
     }
     if (role == 0) {
         J.err[path] = P.gs->code;
@@ -864,15 +963,16 @@ __device__ __forceinline__ void finish_path(int role, const Job &J, PathState &P
 }
 
 // ------------------------------------------------------------------------------------------
-// One solve for the current phase, then the transition.
+// One solve for the current phase, then the transition.  8 lanes: FP64 or FP32 (R); 16 lanes:
+// FP64.
 // ------------------------------------------------------------------------------------------
-template <typename R>
-__device__ __forceinline__ void micro_step(int role, const Job &J, PathState &P, GroupLds<R> *L) {
+template <class G, typename R>
+__device__ __forceinline__ void micro_step(int role, const Job &J, PathState<G> &P, void *lds) {
     const TrackOpts &o = J.o;
     // evaluation point, w row and right-hand side of the current phase
     const bool newton = (P.phase == PH_NEWTON || P.phase == PH_SHARP);
     const bool rk = (P.phase == PH_RK);
-    cx<double> Ye[3], W[3];
+    cx<double> Ye[G::kS], W[G::kS];
     double w_rhs = 1.0;
     {
         double c[7];
@@ -880,7 +980,7 @@ __device__ __forceinline__ void micro_step(int role, const Job &J, PathState &P,
 #pragma unroll
         for (int m = 0; m < 7; ++m) c[m] = rk ? kA[P.stage & 7][m] * s : 0.0;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < G::kS; ++k) {
             cx<double> acc = P.Yc[k];
 #pragma unroll
             for (int m = 0; m < 7; ++m) acc = acc + c[m] * P.K[m][k];
@@ -889,49 +989,53 @@ __device__ __forceinline__ void micro_step(int role, const Job &J, PathState &P,
             cx<double> wv = P.vc[k];
 #pragma unroll
             for (int m = 0; m < 7; ++m) wv = sel(rk && P.stage == m + 1, P.K[m][k], wv);
-            const bool is_t = (role == 7 && k == 1);
-            W[k] = sel(newton, mk(is_t ? 1.0 : 0.0, 0.0), wv);
+            W[k] = sel(newton, mk(is_t_slot<G>(role, k) ? 1.0 : 0.0, 0.0), wv);
         }
     }
     double tcur = 0.0;
     if (newton) {
-        tcur = grp_t(role, Ye);
+        tcur = grp_t<G>(role, Ye);
         w_rhs = P.tgoal - tcur;
     }
-    // FP64 residual of this lane's equations (Newton only)
-    cx<double> fN[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
-    if (newton && sizeof(R) != sizeof(double)) {
-        // gather the point in FP64 through LDS (two cells per value)
-        double *dcell = reinterpret_cast<double *>(&L->phat[0][0]);
-        lds_sync();
-        dcell[4 * role + 0] = Ye[0].re;
-        dcell[4 * role + 1] = Ye[0].im;
-        dcell[4 * role + 2] = Ye[1].re;
-        dcell[4 * role + 3] = Ye[1].im;
-        lds_sync();
-        PoseEval<double> e;
+    cx<double> x[G::kS];
+    if constexpr (G::kW == 16) {
+        group_solve16(role, static_cast<GroupLds16 *>(lds), Ye, P.u, W, newton, w_rhs, P.par, P.gs->gamma, x);
+    } else {
+        GroupLds<R> *L = static_cast<GroupLds<R> *>(lds);
+        // FP64 residual of this lane's equations (Newton with FP32 rows only)
+        cx<double> fN[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+        if (newton && sizeof(R) != sizeof(double)) {
+            // gather the point in FP64 through LDS (two cells per value)
+            double *dcell = reinterpret_cast<double *>(&L->phat[0][0]);
+            lds_sync();
+            dcell[4 * role + 0] = Ye[0].re;
+            dcell[4 * role + 1] = Ye[0].im;
+            dcell[4 * role + 2] = Ye[1].re;
+            dcell[4 * role + 3] = Ye[1].im;
+            lds_sync();
+            PoseEval<double> e;
 #pragma unroll
-        for (int i = 0; i < 14; ++i) e.L[i] = mk(dcell[2 * i], dcell[2 * i + 1]);
-        e.Z0 = mk(dcell[28], dcell[29]);
-        e.t = dcell[30];
-        lds_sync();
-        e.U = role < 7 ? Ye[2] : mk(1.0, 0.0);
+            for (int i = 0; i < 14; ++i) e.L[i] = mk(dcell[2 * i], dcell[2 * i + 1]);
+            e.Z0 = mk(dcell[28], dcell[29]);
+            e.t = dcell[30];
+            lds_sync();
+            e.U = role < 7 ? Ye[2] : mk(1.0, 0.0);
 #pragma unroll
-        for (int p = 0; p < 6; ++p) {
-            e.s[p] = P.par[p];
-            e.d[p] = P.par[6 + p];
+            for (int p = 0; p < 6; ++p) {
+                e.s[p] = P.par[p];
+                e.d[p] = P.par[6 + p];
+            }
+            e.gamma = P.gs->gamma;
+            pose_values(e, fN[0], fN[1], fN[2]);
         }
-        e.gamma = P.gs->gamma;
-        pose_values(e, fN[0], fN[1], fN[2]);
+        group_solve<R>(role, L, Ye, P.u, W, newton, w_rhs, P.par, P.gs->gamma, fN, x);
     }
-    cx<double> x[3];
-    group_solve<R>(role, L, Ye, P.u, W, newton, w_rhs, P.par, P.gs->gamma, fN, x);
     if (!newton) {
         P.gs->st_tan++;
-        normalize_tangent(role, x, P.tstep && P.phase != PH_INIT);
+        normalize_tangent<G>(role, x, P.tstep && P.phase != PH_INIT);
         if (P.phase == PH_INIT) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) P.vc[k] = x[k];
+            for (int k = 0; k < G::kS; ++k) P.vc[k] = x[k];
             main_loop_top(o, P);
             return;
         }
@@ -939,40 +1043,45 @@ __device__ __forceinline__ void micro_step(int role, const Job &J, PathState &P,
 #pragma unroll
         for (int m = 0; m < 8; ++m)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) P.K[m][k] = sel(P.stage == m, x[k], P.K[m][k]);
+            for (int k = 0; k < G::kS; ++k) P.K[m][k] = sel(P.stage == m, x[k], P.K[m][k]);
         P.stage++;
         if (P.stage == 8) finish_rk(role, o, P);
         return;
     }
     // Newton update (htpysupport.cu:475-491)
 #pragma unroll
-    for (int k = 0; k < 3; ++k) P.K[0][k] = P.K[0][k] + x[k];
-    P.residual = grp_norm(x);
+    for (int k = 0; k < G::kS; ++k) P.K[0][k] = P.K[0][k] + x[k];
+    P.residual = grp_norm<G>(x);
     P.its++;
     P.gs->st_newton_its++;
     const int max_its = P.phase == PH_SHARP ? o.sharp_its : o.newt_its;
     const double tol = P.phase == PH_SHARP ? o.sharp_tol : o.newt_tol;
     if (P.its < max_its && P.residual > tol) return;  // another iteration
+    float *xarea;
+    if constexpr (G::kW == 16) xarea = reinterpret_cast<float *>(static_cast<GroupLds16 *>(lds)->a);
+    else xarea = reinterpret_cast<float *>(static_cast<GroupLds<double> *>(lds)->a);
     if (P.phase == PH_SHARP) finish_sharp(role, J, P);
-    else finish_newton(role, J, P, reinterpret_cast<float *>(reinterpret_cast<GroupLds<double> *>(L)->a));
+    else finish_newton(role, J, P, xarea);
 }
 
 // Next path for a persistent group: lane 0 takes a ticket from the device queue.
+template <class G>
 __device__ __forceinline__ int64_t claim_path(int role, unsigned long long *queue) {
     unsigned long long v = 0;
     if (role == 0) v = atomicAdd(queue, 1ull);
-    const int lo = grp_bcast_i((int)(uint32_t)v, role, 0);
-    const int hi = grp_bcast_i((int)(uint32_t)(v >> 32), role, 0);
+    const int lo = grp_bcast_i<G::kW>((int)(uint32_t)v, role, 0);
+    const int hi = grp_bcast_i<G::kW>((int)(uint32_t)(v >> 32), role, 0);
     return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
 // Wave-uniform precision choice for mixed mode: the RK stage tangents are solved in FP32 unless
 // some group of the wave needs FP64 (initial tangent, any Newton step -- corrector and
 // sharpening -- or a stage within fp64_band of t = 1).
-__device__ __forceinline__ bool needs_fp64(int role, const TrackOpts &o, const PathState &P) {
+template <class G>
+__device__ __forceinline__ bool needs_fp64(int role, const TrackOpts &o, const PathState<G> &P) {
     if (o.precision == 0) return true;
     if (P.phase != PH_RK) return true;
-    const double tc = grp_t(role, P.Yc);
+    const double tc = grp_t<G>(role, P.Yc);
     return (1.0 - tc) < o.fp64_band;
 }
 
@@ -980,24 +1089,26 @@ __device__ __forceinline__ bool needs_fp64(int role, const TrackOpts &o, const P
 // Kernels.  Static: group g of the grid tracks path g (the reference's one-thread-per-path
 // launch, frg.cu:374).  Persistent: a fixed grid; groups take paths from the queue until it
 // runs dry.  With learned step control a group without a path stays in the loop (idle) until
-// its whole wave is idle, because the model is evaluated by all 64 lanes of the wave.
+// its whole wave is idle, because the model is evaluated by all 64 lanes of the wave.  The
+// 16-lane layout (4 paths per wave) is built for two waves per SIMD: its register budget is 256.
 // ------------------------------------------------------------------------------------------
-template <bool kPersistent, bool kMixed>
-__global__ __launch_bounds__(256) void track_kernel(Job J) {
+template <bool kPersistent, bool kMixed, class G>
+__device__ __forceinline__ void track_body(const Job &J) {
     extern __shared__ unsigned char smem[];
-    const int role = threadIdx.x & 7;
-    const int grp = threadIdx.x >> 3;
-    GroupSlot *S = reinterpret_cast<GroupSlot *>(smem) + grp;
-    GroupLds<double> *L = &S->lds;
-    // model scratch: the LDS areas of the wave's eight groups
-    float *xarea0 = reinterpret_cast<float *>((S - (grp & 7))->lds.a);
-    constexpr int kXStride = (int)(sizeof(GroupSlot) / sizeof(float));
-    PathState P;
-    P.par = S->par[role];
+    constexpr int kGroupsPerWave = 64 / G::kW;
+    const int role = threadIdx.x & (G::kW - 1);
+    const int grp = threadIdx.x / G::kW;
+    typename G::Slot *S = reinterpret_cast<typename G::Slot *>(smem) + grp;
+    // model scratch: the LDS areas of the wave's groups
+    float *xarea0 = reinterpret_cast<float *>((S - (grp % kGroupsPerWave))->lds.a);
+    constexpr int kXStride = (int)(sizeof(typename G::Slot) / sizeof(float));
+    PathState<G> P;
+    if constexpr (G::kW == 8) P.par = S->par[role];
+    else P.par = S->par;
     P.vc = S->vc[role];
     P.u = S->u[role];
     P.gs = &S->gs;
-    int64_t path = kPersistent ? claim_path(role, J.queue) : (int64_t)blockIdx.x * (blockDim.x >> 3) + grp;
+    int64_t path = kPersistent ? claim_path<G>(role, J.queue) : (int64_t)blockIdx.x * (blockDim.x / G::kW) + grp;
     bool live = path < J.n_total;
     if (!live && !J.learned) return;
     if (live) init_path(role, J, P, path);
@@ -1006,27 +1117,27 @@ __global__ __launch_bounds__(256) void track_kernel(Job J) {
             if (kMixed) {
                 const bool need64 = needs_fp64(role, J.o, P);
                 if (__any(need64)) {
-                    micro_step<double>(role, J, P, L);
+                    micro_step<G, double>(role, J, P, &S->lds);
                 } else {
                     P.gs->st_fp32++;
-                    micro_step<float>(role, J, P, reinterpret_cast<GroupLds<float> *>(L));
+                    micro_step<G, float>(role, J, P, &S->lds);
                 }
             } else {
-                micro_step<double>(role, J, P, L);
+                micro_step<G, double>(role, J, P, &S->lds);
             }
         }
         if (J.learned) {
             const bool pend = live && P.pending;
             const uint64_t lanes = __ballot(pend && role == 0);
             if (lanes) {
-                const float y = mlp_eval_wave(J.model, xarea0, kXStride, lanes, grp & 7, role);
+                const float y = mlp_eval_wave<G::kW>(J.model, xarea0, kXStride, lanes, grp % kGroupsPerWave, role);
                 if (pend) finish_proposal(J.o, P, y);
             }
         }
         if (live && P.done) {
             finish_path(role, J, P);
             if (kPersistent) {
-                path = claim_path(role, J.queue);
+                path = claim_path<G>(role, J.queue);
                 live = path < J.n_total;
                 if (live) init_path(role, J, P, path);
             } else {
@@ -1039,6 +1150,17 @@ __global__ __launch_bounds__(256) void track_kernel(Job J) {
             break;
         }
     }
+}
+
+template <bool kPersistent, bool kMixed>
+__global__ __launch_bounds__(256) void track_kernel(Job J) {
+    track_body<kPersistent, kMixed, Lanes8>(J);
+}
+
+// kWaves: register budget as resident waves per SIMD (1: up to 512 VGPRs, 2: up to 256).
+template <bool kPersistent, int kWaves>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kWaves, kWaves))) void track_kernel16(Job J) {
+    track_body<kPersistent, false, Lanes16>(J);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1142,8 +1264,8 @@ __global__ __launch_bounds__(64) void refine_kernel(Job J) {
             const cx<double> fN[3] = {mk(fI.re.hi, fI.im.hi), mk(fII.re.hi, fII.im.hi), mk(fIII.re.hi, fIII.im.hi)};
             cx<double> x[3];
             group_solve<double, true>(role, L, Y, uu, ww, true, 0.0, par, gamma, fN, x);
-            const double r = grp_norm(x);
-            if (!(r <= kPolishCap * fmax(1.0, grp_norm(Y)))) break;
+            const double r = grp_norm<Lanes8>(x);
+            if (!(r <= kPolishCap * fmax(1.0, grp_norm<Lanes8>(Y)))) break;
 #pragma unroll
             for (int k = 0; k < 3; ++k) Y[k] = Y[k] + x[k];
             ++taken;
@@ -1187,16 +1309,25 @@ hipError_t launch_refine(const Job &job, hipStream_t stream) {
     return hipGetLastError();
 }
 
-size_t track_lds_bytes(int block, int precision) {
-    (void)precision;
-    return (size_t)(block / 8) * sizeof(GroupSlot);
+size_t track_lds_bytes(int block, int lanes) {
+    return lanes == 16 ? (size_t)(block / 16) * sizeof(GroupSlot16) : (size_t)(block / 8) * sizeof(GroupSlot);
 }
 
-int track_max_groups_per_cu(int block, int kind, int precision) {
+int track_max_groups_per_cu(int block, int kind, int precision, int lanes, int waves) {
     int nb = 0;
-    const size_t lds = track_lds_bytes(block, precision);
+    const size_t lds = track_lds_bytes(block, lanes);
     hipError_t e;
-    if (kind == LK_PERSISTENT) {
+    const bool pers = kind == LK_PERSISTENT;
+    if (lanes == 16) {
+        if (precision) return 0;
+        if (waves == 2) {
+            e = pers ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, track_kernel16<true, 2>, block, lds)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, track_kernel16<false, 2>, block, lds);
+        } else {
+            e = pers ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, track_kernel16<true, 1>, block, lds)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, track_kernel16<false, 1>, block, lds);
+        }
+    } else if (pers) {
         e = precision ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, track_kernel<true, true>, block, lds)
                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, track_kernel<true, false>, block, lds);
     } else {
@@ -1204,14 +1335,26 @@ int track_max_groups_per_cu(int block, int kind, int precision) {
                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, track_kernel<false, false>, block, lds);
     }
     if (e != hipSuccess) return 0;
-    return nb * (block / 8);
+    return nb * (block / lanes);
 }
 
-hipError_t launch_track(const Job &job, int kind, int block, int grid, hipStream_t stream) {
+hipError_t launch_track(const Job &job, int kind, int block, int grid, int lanes, int waves, hipStream_t stream) {
     if (block % 64 != 0 || block < 64 || block > 256) return hipErrorInvalidValue;
-    const size_t lds = track_lds_bytes(block, job.o.precision);
+    if (lanes != 8 && lanes != 16) return hipErrorInvalidValue;
+    if (waves != 1 && !(waves == 2 && lanes == 16)) return hipErrorInvalidValue;
+    const size_t lds = track_lds_bytes(block, lanes);
     const bool mixed = job.o.precision != 0;
-    if (kind == LK_PERSISTENT) {
+    if (lanes == 16) {
+        if (mixed) return hipErrorInvalidValue;  // the 16-lane solve is FP64 only
+        const bool pers = kind == LK_PERSISTENT;
+        if (waves == 2) {
+            if (pers) hipLaunchKernelGGL((track_kernel16<true, 2>), dim3(grid), dim3(block), lds, stream, job);
+            else hipLaunchKernelGGL((track_kernel16<false, 2>), dim3(grid), dim3(block), lds, stream, job);
+        } else {
+            if (pers) hipLaunchKernelGGL((track_kernel16<true, 1>), dim3(grid), dim3(block), lds, stream, job);
+            else hipLaunchKernelGGL((track_kernel16<false, 1>), dim3(grid), dim3(block), lds, stream, job);
+        }
+    } else if (kind == LK_PERSISTENT) {
         if (mixed) hipLaunchKernelGGL((track_kernel<true, true>), dim3(grid), dim3(block), lds, stream, job);
         else hipLaunchKernelGGL((track_kernel<true, false>), dim3(grid), dim3(block), lds, stream, job);
     } else {
@@ -1272,7 +1415,7 @@ __global__ __launch_bounds__(64) void debug_solve_kernel(DebugSolveArgs A) {
     if (newton) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) W[k] = mk((role == 7 && k == 1) ? 1.0 : 0.0, 0.0);
-        w_rhs = A.tgoal[p] - grp_t(role, Y);
+        w_rhs = A.tgoal[p] - grp_t<Lanes8>(role, Y);
         // FP64 residual values of this lane's equations
         PoseEval<double> e;
         const cx<double> *y = A.Y + p * 23;
@@ -1290,8 +1433,46 @@ __global__ __launch_bounds__(64) void debug_solve_kernel(DebugSolveArgs A) {
         pose_values(e, fN[0], fN[1], fN[2]);
     }
     group_solve<R>(role, L, Y, U, W, newton, w_rhs, par, A.gamma, fN, x);
-    if (!newton) normalize_tangent(role, x, A.mode == 1);
+    if (!newton) normalize_tangent<Lanes8>(role, x, A.mode == 1);
     store_slots(role, A.x + p * 23, x);
+}
+
+// The same with one path per 16-lane group (FP64).
+__global__ __launch_bounds__(64) void debug_solve16_kernel(DebugSolveArgs A) {
+    extern __shared__ unsigned char smem[];
+    const int role = threadIdx.x & 15;
+    const int grp = threadIdx.x >> 4;
+    GroupSlot16 *S = reinterpret_cast<GroupSlot16 *>(smem) + grp;
+    const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 4) + grp;
+    if (p >= A.n) return;
+    const cx<double> *sp = A.spm + p * 30;
+    for (int i = role; i < 30; i += 16) {
+        S->par[2 * i] = sp[i];
+        S->par[2 * i + 1] = A.fpm[i] - sp[i];
+    }
+    cx<double> Y[2], U[2], W[2], x[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int yi = Lanes16::slot_index(role, k);
+        Y[k] = yi < 0 ? mk(0.0, 0.0) : A.Y[p * 23 + yi];
+        U[k] = yi < 0 ? mk(0.0, 0.0) : A.u[p * 23 + yi];
+        W[k] = yi < 0 ? mk(0.0, 0.0) : A.w[p * 23 + yi];
+    }
+    lds_sync();
+    const bool newton = A.mode == 2;
+    double w_rhs = 1.0;
+    if (newton) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) W[k] = mk(is_t_slot<Lanes16>(role, k) ? 1.0 : 0.0, 0.0);
+        w_rhs = A.tgoal[p] - grp_t<Lanes16>(role, Y);
+    }
+    group_solve16(role, &S->lds, Y, U, W, newton, w_rhs, S->par, A.gamma, x);
+    if (!newton) normalize_tangent<Lanes16>(role, x, A.mode == 1);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int yi = Lanes16::slot_index(role, k);
+        if (yi >= 0) A.x[p * 23 + yi] = x[k];
+    }
 }
 
 __global__ __launch_bounds__(64) void debug_start_kernel(DebugStartArgs A) {
@@ -1299,34 +1480,40 @@ __global__ __launch_bounds__(64) void debug_start_kernel(DebugStartArgs A) {
     const int grp = threadIdx.x >> 3;
     const int64_t p = (int64_t)blockIdx.x * (blockDim.x >> 3) + grp;
     if (p >= A.n) return;
-    PathState P;
-    cx<double> parbuf[12];
+    PathState<Lanes8> P;
     GroupScalars gsbuf;
-    P.par = parbuf;
     P.gs = &gsbuf;
     P.gs->seed = A.seed;
     P.gs->pid = A.pid0 + (uint64_t)p;
-    generate_start(role, A.o, P);
+    const int j = lane_pose<Lanes8>(role);
+    StartPose sp;
+    start_pose(A.o, P, j, sp);
     cx<double> *z = A.spt + p * 21;
     cx<double> *q = A.spm + p * 30;
     if (role < 7) {
-        const int j = role + 1;
-        z[2 * role] = P.Yc[0];
-        z[2 * role + 1] = P.Yc[1];
-        z[14 + role] = P.Yc[2];
-        q[j] = P.par[1];
-        q[8 + j] = P.par[3];
-        q[15 + j] = P.par[4];
-        q[22 + j] = P.par[5];
+        z[2 * role] = sp.z[2 * role];
+        z[2 * role + 1] = sp.z[2 * role + 1];
+        z[14 + role] = sp.U;
+        q[j] = sp.q[1];
+        q[8 + j] = sp.q[3];
+        q[15 + j] = sp.q[4];
+        q[22 + j] = sp.q[5];
     }
     if (role == 0) {
-        q[0] = P.par[0];
-        q[8] = P.par[2];
+        q[0] = sp.q[0];
+        q[8] = sp.q[2];
     }
 }
 
-hipError_t launch_debug_solve(const DebugSolveArgs &a, int precision, hipStream_t stream) {
+hipError_t launch_debug_solve(const DebugSolveArgs &a, int precision, int lanes, hipStream_t stream) {
     const int block = 64;
+    if (lanes == 16) {
+        if (precision) return hipErrorInvalidValue;
+        const int grid = (int)((a.n + 3) / 4);
+        if (grid == 0) return hipSuccess;
+        hipLaunchKernelGGL(debug_solve16_kernel, dim3(grid), dim3(block), 4 * sizeof(GroupSlot16), stream, a);
+        return hipGetLastError();
+    }
     const int grid = (int)((a.n + 7) / 8);
     if (grid == 0) return hipSuccess;
     if (precision) {

@@ -72,9 +72,13 @@ typedef struct {
     int32_t groups_per_cu;                 /* persistent grid: paths in flight per CU; 0 = occupancy */
     int32_t step_control;                  /* FRG_STEP_* */
     int32_t use_graph;                     /* 1: launch through a cached hipGraph (queue reset + tracker) */
-    int32_t lanes_per_path;                /* lanes of a wave64 that track one path: 8 or 16; 0 = default (8) */
+    int32_t lanes_per_path;                /* lanes of a wave64 that track one path: 8 (8 paths per wave) or 16
+                                              (4 paths per wave, FP64 only); 0 = default (8) */
     int32_t polish_its;                    /* FP64 Newton iterations on a sharpened root in the dehomogenising
                                               patch e_21 (frg.cu:44-46) at t = 1; 0 = none, as the reference */
+    int32_t waves_per_simd;                /* register budget of the 16-lane tracker as resident waves per SIMD:
+                                              1 (up to 512 VGPRs per lane) or 2 (256); 0 = 1.  Only 1 with 8 lanes */
+    int32_t pad0;
 } frg_opts;
 
 /* Per-call summary. */
@@ -86,6 +90,7 @@ typedef struct {
     double total_ms;        /* host wall time of the whole call, copies included */
     int32_t block_size, grid_size, schedule, precision;
     double refine_ms;       /* device time of the refinement + cognate launch that follows it */
+    int32_t lanes_per_path, waves_per_simd;
 } frg_stats;
 
 /* Opaque context: device, stream, device buffers, tuning cache. */
@@ -158,16 +163,20 @@ int frg_track_batch_noalloc(frg_ctx *ctx, int32_t n_inst, const frg_c64 *d_fpm, 
 /* One measured launch shape of frg_autotune. */
 typedef struct {
     int32_t schedule, block_size, groups_per_cu; /* groups_per_cu: resident paths per CU (persistent) */
+    int32_t lanes_per_path;                      /* lanes per path: paths per wave = 64 / lanes_per_path */
+    int32_t waves_per_simd, pad0;                /* register budget (frg_opts.waves_per_simd) */
     double kernel_ms, paths_per_s;
 } frg_tune_entry;
 
-/* Times the tracker on n_sample paths (RNG ids first_path_id + i of `seed`, target fpm/gamma) for
- * each candidate launch shape -- static or persistent schedule, 64/128/256-thread blocks, and for
- * the persistent grid 1..max blocks per CU -- with the context's precision and step control, then
- * sets the context's schedule, block_size and groups_per_cu to the fastest.  Up to table_cap
- * measurements go to table (may be NULL), their number to *n_entries.  Paths per wave are fixed at
- * 8 by the tracker's lane mapping.  No reference counterpart (the reference launches 80 x 256,
- * frg.cu:25-27). */
+/* Times the tracking kernel on n_sample paths (RNG ids first_path_id + i of `seed`, target
+ * fpm/gamma) for each candidate launch shape -- paths per wave (8 lanes per path, or 16 for FP64
+ * with a register budget of 1 or 2 waves per SIMD),
+ * static or persistent schedule, 64/128/256-thread blocks, and for the persistent grid 1..max
+ * blocks per CU -- with the context's precision and step control, then sets the context's
+ * lanes_per_path, waves_per_simd, schedule, block_size and groups_per_cu to the fastest.  Up to table_cap
+ * measurements go to table (may be NULL), their number to *n_entries.  A sample of a few hundred
+ * thousand paths keeps the launch's tail (its slowest paths) a small part of each timing.  No
+ * reference counterpart (the reference launches 80 x 256, frg.cu:25-27). */
 int frg_autotune(frg_ctx *ctx, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
                  int64_t n_sample, frg_tune_entry *table, int32_t table_cap, int32_t *n_entries);
 
@@ -179,7 +188,8 @@ int frg_generate_starts(frg_ctx *ctx, uint64_t seed, uint64_t first_path_id, int
  *   mode 1: IVP with tstepQ = true;
  *   mode 2: one Newton update toward tgoal[i] (htpysupport.cu:430-480; w ignored).
  * Y, u, w: n x 23; spm: n x 30; fpm: 30; x: n x 23 out.  precision FRG_PREC_FP64 or
- * FRG_PREC_MIXED (FP32 Jacobian and solve, FP64 residual). */
+ * FRG_PREC_MIXED (FP32 Jacobian and solve, FP64 residual).  The solve is the tracker's for the
+ * context's lanes_per_path (16: FP64 only, FRG_EUNSUPPORTED otherwise). */
 int frg_solve_step(frg_ctx *ctx, int32_t mode, int32_t precision, int64_t n, const frg_c64 *Y, const frg_c64 *u,
                    const frg_c64 *w, const frg_c64 *spm, const frg_c64 *fpm, frg_c64 gamma, const double *tgoal,
                    frg_c64 *x);

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from finite_root_generation_amd import capi, tasks
+from tests.rootcmp import ROOT_TOL, abs_diff
 
 pytestmark = pytest.mark.gpu
 
@@ -53,17 +54,28 @@ def test_autotune_keeps_the_fastest_shape():
         say("[gpu] autotune on 8192 paths")
         rows, chosen = t.autotune(fpm, gamma, seed=3, n_sample=8192, first_path_id=1 << 40)
         for r in rows:
-            say(f"    sched {r['schedule']} block {r['block_size']} groups/CU {r['groups_per_cu']}: "
+            say(f"    sched {r['schedule']} block {r['block_size']} groups/CU {r['groups_per_cu']} "
+                f"lanes {r['lanes_per_path']} waves/SIMD {r['waves_per_simd']}: "
                 f"{r['kernel_ms']:.1f} ms, {r['paths_per_s']:.0f} paths/s")
         assert len(rows) >= 4
+        assert {r["lanes_per_path"] for r in rows} == {8, 16}  # FP64: both path widths are candidates
         best = min(rows, key=lambda r: r["kernel_ms"])
-        assert chosen == (best["schedule"], best["block_size"], best["groups_per_cu"])
-        # the chosen shape tracks the same paths as the default launch
+        assert chosen == (best["schedule"], best["block_size"], best["groups_per_cu"], best["lanes_per_path"],
+                          best["waves_per_simd"])
+        # the chosen shape tracks the same paths as the default launch: bit for bit with the same
+        # path width (the schedule does not change a path's arithmetic), to the root tolerance
+        # with the other width (the 16-lane solve sums in another order)
         a = t.track(fpm, gamma, seed=9, n=256)
         with capi.Tracker(0) as d:
             b = d.track(fpm, gamma, seed=9, n=256)
-        assert np.array_equal(a[1], b[1])
-        assert np.array_equal(a[0].view(np.float64), b[0].view(np.float64))
+        if chosen[3] == 8:
+            assert np.array_equal(a[1], b[1])
+            assert np.array_equal(a[0].view(np.float64), b[0].view(np.float64))
+        else:
+            both = tasks.finite_mask(*a) & tasks.finite_mask(*b)
+            same = both & (abs_diff(a[0], b[0]) <= ROOT_TOL)
+            say(f"    16 vs 8 lanes: {int(same.sum())} of {int(both.sum())} common finite roots agree")
+            assert same.sum() >= 0.98 * both.sum() and (a[1] == b[1]).mean() >= 0.98
     finally:
         t.close()
 
