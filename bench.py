@@ -217,8 +217,9 @@ def chosen_opts(capi):
                 start_mode=capi.START_GAUSSIAN)
 
 
-def traffic_probe(n):
-    """One chosen launch at full size, then a 1 GiB device copy (counter calibration)."""
+def traffic_probe(n, lanes=8):
+    """One chosen launch at full size (lanes: path width, 8 or 16 lanes per path), then a 1 GiB
+    device copy (counter calibration)."""
     import torch
 
     from finite_root_generation_amd import capi, tasks
@@ -230,9 +231,9 @@ def traffic_probe(n):
     fpt = torch.empty((n, 21), dtype=torch.complex128, device=dev)
     err = torch.empty((n,), dtype=torch.int32, device=dev)
     st = torch.empty((n, 8), dtype=torch.int32, device=dev)
-    with capi.Tracker(0, **chosen_opts(capi)) as t:
+    with capi.Tracker(0, lanes_per_path=lanes, **chosen_opts(capi)) as t:
         t.set_step_model(step_model)
-        log("[probe] chosen launch")
+        log(f"[probe] chosen launch, {lanes} lanes per path")
         t.track_device(fpm, gamma, 2, n, fpt.data_ptr(), err.data_ptr(), st.data_ptr())
         torch.cuda.synchronize()
         log(f"[probe] kernel {t.last_kernel_ms():.1f} ms")
@@ -347,6 +348,7 @@ def main():
     ap.add_argument("--no-batched-plain", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="chosen launch shape without the autotuner")
     ap.add_argument("--traffic-probe", action="store_true")
+    ap.add_argument("--probe-lanes", type=int, default=8, choices=(8, 16), help="path width of the traffic probe")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "bench.py measures one GPU; more GPUs would each track a contiguous "
@@ -354,7 +356,7 @@ def main():
         sys.exit(2)
 
     if args.traffic_probe:
-        traffic_probe(args.paths)
+        traffic_probe(args.paths, args.probe_lanes)
         return
 
     import torch

@@ -2,9 +2,11 @@
 //
 //  * cx<R>       complex number over R = double or float ({re, im}, bit-compatible with the
 //                reference's Cplx for R = double, classes.cuh:26-33).
-//  * group ops   8-lane groups inside a wave64 (one path per group): butterfly sums, maxima,
-//                arg-max and broadcasts built on DPP (quad_perm and row_half_mirror), so every
-//                lane of a group ends with bitwise identical results.
+//  * group ops   8- or 16-lane groups inside a wave64 (one path per group): butterfly sums,
+//                maxima, arg-max and broadcasts built on DPP (quad_perm, row_half_mirror and, for
+//                16 lanes, row_mirror), so every lane of a group ends with bitwise identical
+//                results; pair ops between lanes 2i and 2i+1.
+//  * dd          double-double arithmetic (error-free transformations) for extended residuals.
 //  * Philox4x32-10 (Salmon et al., SC'11) counter-based RNG.
 #pragma once
 

@@ -320,21 +320,31 @@ struct GroupSlot {
     cx<double> vc[8][3];    // per lane: its slots of the current velocity Vc
     cx<double> u[8][3];     // per lane: its slots of the projective patch u
     GroupScalars gs;
+    // The groups of a wave read the same field of their own slots together (a broadcast within
+    // each group): with a slot size that is an odd multiple of 16 bytes those 16-byte reads fall
+    // in disjoint LDS banks (8 groups x 16 B over 32 or 64 four-byte banks).  5,072 bytes: 32
+    // slots still fit the 160 KB of a CU.
+    unsigned char bank_pad[16];
 };
 static_assert(sizeof(GroupSlot) % 16 == 0 && offsetof(GroupLds<double>, a) % 16 == 0,
               "the step model reads float4 vectors from GroupLds::a");
+static_assert((sizeof(GroupSlot) / 16) % 2 == 1, "slot stride: an odd number of 16-byte bank rows");
+static_assert(32 * sizeof(GroupSlot) <= 160 * 1024, "32 groups (256 threads) per CU");
 static_assert(sizeof(GroupLds<double>::a) >= 128 * sizeof(float), "features (8) and hidden layer (64 at +64)");
 
-// Per-group LDS of the 16-lane layout (4,736 bytes: 32 paths on a CU at two waves per SIMD).
+// Per-group LDS of the 16-lane layout (4,752 bytes: 32 paths on a CU at two waves per SIMD).
 struct GroupSlot16 {
     GroupLds16 lds;
     cx<double> par[60];    // the 30 parameters as (start value, target - start) pairs
     cx<double> vc[16][2];  // per lane: its slots of the current velocity Vc
     cx<double> u[16][2];   // per lane: its slots of the projective patch u
     GroupScalars gs;
+    unsigned char bank_pad[16];  // odd number of 16-byte bank rows, as GroupSlot
 };
 static_assert(sizeof(GroupSlot16) % 16 == 0 && offsetof(GroupLds16, a) % 16 == 0,
               "the step model reads float4 vectors from GroupLds16::a");
+static_assert((sizeof(GroupSlot16) / 16) % 2 == 1, "slot stride: an odd number of 16-byte bank rows");
+static_assert(32 * sizeof(GroupSlot16) <= 160 * 1024, "32 groups per CU at two waves per SIMD");
 static_assert(sizeof(GroupLds16::a) >= 128 * sizeof(float), "features (8) and hidden layer (64 at +64)");
 
 // Lane layouts.  kW lanes track one path; each lane holds kS slots of every 23-vector
