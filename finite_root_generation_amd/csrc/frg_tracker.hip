@@ -755,6 +755,8 @@ __device__ __forceinline__ void finish_rk(int role, const TrackOpts &o, PathStat
         P.phase = PH_RK;
         P.stage = 1;
         clear_stages<1>(P);
+#pragma unroll
+        for (int k = 0; k < G::kS; ++k) P.K[7][k] = P.K[0][k];  // the second stage's w row is k1
         return;
     }
     const double tn = grp_t<G>(role, P.K[1]);
@@ -989,17 +991,16 @@ __device__ __forceinline__ void micro_step(int role, const Job &J, PathState<G> 
         const double s = P.s;
 #pragma unroll
         for (int m = 0; m < 7; ++m) c[m] = rk ? kA[P.stage & 7][m] * s : 0.0;
+        const bool w_last = rk && P.stage > 0;
 #pragma unroll
         for (int k = 0; k < G::kS; ++k) {
             cx<double> acc = P.Yc[k];
 #pragma unroll
             for (int m = 0; m < 7; ++m) acc = acc + c[m] * P.K[m][k];
             Ye[k] = sel(newton, P.K[0][k], acc);
-            // w: Vc at init and stage 0, k_{stage-1} otherwise; e_t for Newton
-            cx<double> wv = P.vc[k];
-#pragma unroll
-            for (int m = 0; m < 7; ++m) wv = sel(rk && P.stage == m + 1, P.K[m][k], wv);
-            W[k] = sel(newton, mk(is_t_slot<G>(role, k) ? 1.0 : 0.0, 0.0), wv);
+            // w: Vc at init and stage 0, k_{stage-1} (kept in K[7] until stage 8 is stored) for
+            // the later stages, e_t for Newton
+            W[k] = sel(newton, mk(is_t_slot<G>(role, k) ? 1.0 : 0.0, 0.0), sel(w_last, P.K[7][k], P.vc[k]));
         }
     }
     double tcur = 0.0;
@@ -1049,11 +1050,14 @@ __device__ __forceinline__ void micro_step(int role, const Job &J, PathState<G> 
             main_loop_top(o, P);
             return;
         }
-        // store k_{stage+1}
+        // store k_{stage+1}; K[7] holds the latest stage tangent (the next stage's w row) until the
+        // eighth stage is stored there
 #pragma unroll
-        for (int m = 0; m < 8; ++m)
+        for (int m = 0; m < 7; ++m)
 #pragma unroll
             for (int k = 0; k < G::kS; ++k) P.K[m][k] = sel(P.stage == m, x[k], P.K[m][k]);
+#pragma unroll
+        for (int k = 0; k < G::kS; ++k) P.K[7][k] = x[k];
         P.stage++;
         if (P.stage == 8) finish_rk(role, o, P);
         return;
