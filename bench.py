@@ -19,9 +19,10 @@ value = chosen paths/s (wall clock around the timed steps).  One JSON line on st
 The line also carries "batched", the many-instance workload: 64 task instances (seeded tasks
 200..263, start seeds 300..363) with path counts log-uniform in [1,024, 262,144] (numpy seed 4),
 FP32/FP64 mixed precision and learned step control, launch shape picked by the autotuner, all
-instances in one persistent launch replayed from a hipGraph; next to it the plain way, each
-instance its own static FP64 launch with classical control, as the reference runs one target at a
-time.
+instances in one persistent launch replayed from a hipGraph; beside it ("fp64_learned") the same
+launch in FP64 with the chosen tracker's tuned shape, since FP32 solves do not run faster on this
+GPU; and the plain way, each instance its own static FP64 launch with classical control, as the
+reference runs one target at a time.
 
 Parity and recall in the line: the CPU baseline's oracle run (the first paths of the same
 workload) is compared path by path with the plain and chosen GPU runs (codes, roots to 1e-8
@@ -154,19 +155,45 @@ def distinct_roots(capi, res):
     return coll[ok], time.perf_counter() - t0
 
 
-def set_recall(ref_set, other_set):
-    """Fraction of ref_set's roots with a root of other_set within ROOT_TOL (absolute, every
-    component); both sets sorted on Re z0."""
-    if len(ref_set) == 0:
-        return 1.0
+def set_found(ref_set, other_set):
+    """Number of ref_set's roots with a root of other_set within ROOT_TOL (absolute, every
+    component); other_set sorted on Re z0.  Candidates are the other roots whose Re z0 lies within
+    ROOT_TOL, visited by offset from the first one."""
+    if len(ref_set) == 0 or len(other_set) == 0:
+        return 0
     keys = other_set[:, 0].real
-    found = 0
-    for x in ref_set:
-        lo = np.searchsorted(keys, x[0].real - ROOT_TOL, side="left")
-        hi = np.searchsorted(keys, x[0].real + ROOT_TOL, side="right")
-        if hi > lo and np.any(np.max(np.abs(other_set[lo:hi] - x), axis=1) <= ROOT_TOL):
-            found += 1
-    return found / len(ref_set)
+    lo = np.searchsorted(keys, ref_set[:, 0].real - ROOT_TOL, side="left")
+    hi = np.searchsorted(keys, ref_set[:, 0].real + ROOT_TOL, side="right")
+    found = np.zeros(len(ref_set), bool)
+    for j in range(int((hi - lo).max(initial=0))):
+        idx = lo + j
+        live = (idx < hi) & ~found
+        if not live.any():
+            break
+        cand = other_set[np.minimum(idx, len(other_set) - 1)]
+        found |= live & (np.max(np.abs(cand - ref_set), axis=1) <= ROOT_TOL)
+    return int(found.sum())
+
+
+def set_recall(ref_set, other_set):
+    """Fraction of ref_set's roots found in other_set (see set_found)."""
+    return set_found(ref_set, other_set) / len(ref_set) if len(ref_set) else 1.0
+
+
+def instance_sets(capi, tasks, fpt, err, offs):
+    """Distinct finite roots of each instance under the reference's dedupe, sorted on Re z0."""
+    out = []
+    for k in range(len(offs) - 1):
+        coll, _ = capi.sort_dedupe_merge(np.zeros((0, 21)), fpt[offs[k]:offs[k + 1]],
+                                         err[offs[k]:offs[k + 1]].copy(), tol=DEDUPE_TOL)
+        out.append(coll[tasks.finite_mask(coll, np.zeros(len(coll), np.int32))])
+    return out
+
+
+def sets_recall(ref_sets, other_sets):
+    """Pooled recall over instances: roots of ref_sets found in the same instance's other set."""
+    tot = sum(len(r) for r in ref_sets)
+    return sum(set_found(r, o) for r, o in zip(ref_sets, other_sets)) / tot if tot else 1.0
 
 
 def per_path_vs_oracle(tasks, g_fpt, g_err, o_fpt, o_err):
@@ -258,8 +285,32 @@ def tune_table(capi, rows):
              "paths_per_s": round(r["paths_per_s"], 1)} for r in rows]
 
 
-def batched(capi, tasks, step_model, with_plain=True):
-    """64 instances in one graph-replayed persistent launch, autotuned (see the module doc); with
+def batched_run(capi, step_model, fpms, gams, seeds, pids, counts, precision, shape_opts=None):
+    """One graph-replayed persistent launch of all instances (warm-up, then timed).  The launch
+    shape is autotuned unless shape_opts (an frg_opts with a tuned shape) is given."""
+    name = "mixed" if precision == capi.PREC_MIXED else "fp64"
+    with capi.Tracker(0, precision=precision, use_graph=1, step_control=capi.STEP_LEARNED,
+                      start_mode=capi.START_GAUSSIAN) as t:
+        t.set_step_model(step_model)
+        rows = None
+        if shape_opts is None:
+            log(f"[bench] batched {name}: autotune on {TUNE_SAMPLE} paths")
+            rows, chosen = t.autotune(fpms[0], gams[0], seed=9, n_sample=TUNE_SAMPLE, first_path_id=1 << 40)
+        else:
+            t.set_opts(schedule=shape_opts.schedule, block_size=shape_opts.block_size,
+                       groups_per_cu=shape_opts.groups_per_cu, lanes_per_path=shape_opts.lanes_per_path,
+                       waves_per_simd=shape_opts.waves_per_simd)
+        log(f"[bench] batched {name}: shape {shape_of(capi, t.opts)}; warm-up")
+        t.track_batch(fpms, gams, seeds, pids, counts)
+        log(f"[bench] batched {name}: timed")
+        fpt, err, st, summ = t.track_batch(fpms, gams, seeds, pids, counts)
+        shape = shape_of(capi, t.opts)
+    return fpt, err, summ, shape, rows
+
+
+def batched(capi, tasks, step_model, with_plain=True, fp64_shape=None):
+    """64 instances in one graph-replayed persistent launch with learned step control (see the
+    module doc), in mixed precision (autotuned) and in FP64 (the headline's tuned shape); with
     with_plain, also the plain way: each instance its own static FP64 launch, classical control."""
     rng = np.random.default_rng(4)
     counts = np.exp(rng.uniform(np.log(1024), np.log(262144), 64)).astype(np.int64)
@@ -267,16 +318,8 @@ def batched(capi, tasks, step_model, with_plain=True):
     fpms, gams = np.array(fpms), np.array(gams)
     seeds = 300 + np.arange(64)
     pids = np.zeros(64, np.int64)
-    with capi.Tracker(0, precision=capi.PREC_MIXED, use_graph=1, step_control=capi.STEP_LEARNED,
-                      start_mode=capi.START_GAUSSIAN) as t:
-        t.set_step_model(step_model)
-        log(f"[bench] batched: autotune on {TUNE_SAMPLE} paths")
-        rows, chosen = t.autotune(fpms[0], gams[0], seed=9, n_sample=TUNE_SAMPLE, first_path_id=1 << 40)
-        log(f"[bench] batched: chosen {chosen}; warm-up")
-        t.track_batch(fpms, gams, seeds, pids, counts)
-        log("[bench] batched: timed")
-        fpt, err, st, summ = t.track_batch(fpms, gams, seeds, pids, counts)
-        shape = shape_of(capi, t.opts)
+    fpt, err, summ, shape, rows = batched_run(capi, step_model, fpms, gams, seeds, pids, counts, capi.PREC_MIXED)
+    f64 = batched_run(capi, step_model, fpms, gams, seeds, pids, counts, capi.PREC_FP64, shape_opts=fp64_shape)
     n = int(counts.sum())
     plain = None
     if with_plain:
@@ -292,14 +335,37 @@ def batched(capi, tasks, step_model, with_plain=True):
                 if k % 8 == 7:
                     log(f"[bench] batched: plain, {k + 1}/64 instances")
         pf, pe = np.concatenate(pf), np.concatenate(pe)
-        log("[bench] batched: distinct roots and recall against the plain launches")
+        log("[bench] batched: distinct roots per instance and recall against the plain launches")
         ok_c, ok_p = tasks.finite_mask(fpt, err), tasks.finite_mask(pf, pe)
         both = ok_c & ok_p
         same_path = both & (np.max(np.abs(fpt - pf), axis=1) <= ROOT_TOL)
+        offs = np.concatenate([[0], np.cumsum(counts)])
+        sets_p = instance_sets(capi, tasks, pf, pe, offs)
+        sets_m = instance_sets(capi, tasks, fpt, err, offs)
+        sets_f = instance_sets(capi, tasks, f64[0], f64[1], offs)
         plain = {"value": round(n / (plain_ms * 1e-3), 1), "kernel_ms": round(plain_ms, 1),
                  "how": "64 separate launches, static schedule, FP64, classical step control",
-                 "finite_roots": int(ok_p.sum()), "same_root_on_same_path": int(same_path.sum())}
+                 "finite_roots": int(ok_p.sum()), "same_root_on_same_path": int(same_path.sum()),
+                 "distinct_roots": int(sum(len(x) for x in sets_p))}
+        set_stats = {"mixed": {"distinct_roots": int(sum(len(x) for x in sets_m)),
+                               "recall_of_plain_sets": round(sets_recall(sets_p, sets_m), 6),
+                               "recall_by_plain_sets": round(sets_recall(sets_m, sets_p), 6)},
+                     "fp64": {"distinct_roots": int(sum(len(x) for x in sets_f)),
+                              "recall_of_plain_sets": round(sets_recall(sets_p, sets_f), 6),
+                              "recall_by_plain_sets": round(sets_recall(sets_f, sets_p), 6)},
+                     "how": "per instance, the reference's dedupe (1e-6), then roots matched at 1e-8 absolute"}
     ms = summ.kernel_ms + summ.refine_ms
+    f_fpt, f_err, f_summ, f_shape, _ = f64
+    f_ms = f_summ.kernel_ms + f_summ.refine_ms
+    f_ok = tasks.finite_mask(f_fpt, f_err)
+    fp64 = {"value": round(n / (f_ms * 1e-3), 1), "launch_ms": round(f_ms, 1), "precision": "fp64",
+            "step_control": "learned", "graph": True, "shape": f_shape,
+            "finite_roots": int(f_ok.sum()), "failures": np.bincount(f_err, minlength=6).tolist()}
+    if plain:
+        fp64.update(set_stats["fp64"])
+        fp64["speedup_vs_plain"] = round(fp64["value"] / plain["value"], 3)
+        fp64["same_root_on_same_path_as_plain"] = int((f_ok & tasks.finite_mask(pf, pe) & (
+            np.max(np.abs(f_fpt - pf), axis=1) <= ROOT_TOL)).sum())
     return {"instances": 64, "paths": n, "path_counts": {"min": int(counts.min()), "max": int(counts.max())},
             "value": round(n / (ms * 1e-3), 1), "unit": "paths/s", "launch_ms": round(ms, 1),
             "pcie_inclusive_paths_per_s": round(n / (summ.total_ms * 1e-3), 1),
@@ -307,7 +373,8 @@ def batched(capi, tasks, step_model, with_plain=True):
             "autotune": {"chosen": shape, "sample_paths": TUNE_SAMPLE, "candidates": tune_table(capi, rows)},
             "finite_roots": int(tasks.finite_mask(fpt, err).sum()),
             "failures": np.bincount(err, minlength=6).tolist(),
-            "plain": plain, "speedup_vs_plain": round(n / (ms * 1e-3) / plain["value"], 3) if plain else None}
+            "plain": plain, "speedup_vs_plain": round(n / (ms * 1e-3) / plain["value"], 3) if plain else None,
+            "distinct_roots": set_stats["mixed"] if plain else None, "fp64_learned": fp64}
 
 
 def cpu_baseline(fpm, gamma, seed, n_sample, start_mode):
@@ -459,7 +526,8 @@ def main():
         "oracle_comparison": None,
     }
     if not args.no_batched:
-        line["batched"] = batched(capi, tasks, step_model, with_plain=not args.no_batched_plain)
+        line["batched"] = batched(capi, tasks, step_model, with_plain=not args.no_batched_plain,
+                                  fp64_shape=chosen["opts"])
     if not args.no_cpu:
         cores = len(os.sched_getaffinity(0))
         ns = min(n, args.cpu_sample or max(64, 1000 * cores))
