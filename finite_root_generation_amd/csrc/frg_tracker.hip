@@ -244,19 +244,20 @@ __device__ __forceinline__ void group_solve(int role, GroupLds<R> *L, const cx<d
             usedA = true;
             pcA = c;
             invA = ipiv;
-            L->a[c] = ipiv;
-#pragma unroll
-            for (int k = 0; k < 17; ++k)
-                if (k > c) L->a[k] = ra[k];
         }
         if (pB) {
             usedB = true;
             pcB = c;
             invB = ipiv;
+        }
+        // one write branch for either row of the pivot lane: every LDS instruction of the wave
+        // occupies the CU's LDS pipe, so two exec-masked branches of (16 - c) writes each cost
+        // twice the issue time of one
+        if (pA || pB) {
             L->a[c] = ipiv;
 #pragma unroll
             for (int k = 0; k < 17; ++k)
-                if (k > c) L->a[k] = rb[k];
+                if (k > c) L->a[k] = sel(pB, rb[k], ra[k]);
         }
         lds_order();
         const cx<R> ip = L->a[c];
@@ -987,17 +988,22 @@ __device__ __forceinline__ void micro_step(int role, const Job &J, PathState<G> 
     cx<double> Ye[G::kS], W[G::kS];
     double w_rhs = 1.0;
     {
+        // The stage row is read for every phase (stage & 7 is a valid row) and the point chosen
+        // afterwards: a read under the phase condition becomes one branch and one wait for the
+        // load per coefficient, seven round trips to the cache instead of one.
         double c[7];
         const double s = P.s;
+        const double *arow = kA[P.stage & 7];
 #pragma unroll
-        for (int m = 0; m < 7; ++m) c[m] = rk ? kA[P.stage & 7][m] * s : 0.0;
+        for (int m = 0; m < 7; ++m) c[m] = arow[m] * s;
         const bool w_last = rk && P.stage > 0;
 #pragma unroll
         for (int k = 0; k < G::kS; ++k) {
             cx<double> acc = P.Yc[k];
 #pragma unroll
             for (int m = 0; m < 7; ++m) acc = acc + c[m] * P.K[m][k];
-            Ye[k] = sel(newton, P.K[0][k], acc);
+            // initial tangent: the current point itself
+            Ye[k] = sel(newton, P.K[0][k], sel(rk, acc, P.Yc[k]));
             // w: Vc at init and stage 0, k_{stage-1} (kept in K[7] until stage 8 is stored) for
             // the later stages, e_t for Newton
             W[k] = sel(newton, mk(is_t_slot<G>(role, k) ? 1.0 : 0.0, 0.0), sel(w_last, P.K[7][k], P.vc[k]));
