@@ -35,7 +35,8 @@ EXPORTS = ["frg_default_opts", "frg_create", "frg_destroy", "frg_set_opts", "frg
            "frg_track_explicit", "frg_track_batch", "frg_generate_starts", "frg_solve_step",
            "frg_sort_dedupe_merge", "frg_lambertw", "frg_completeness", "frg_strerror", "frg_version",
            "frg_last_kernel_ms", "frg_last_refine_ms", "frg_flops", "frg_collect_step_log", "frg_train_step_model",
-           "frg_set_step_model", "frg_step_model_eval", "frg_step_model_grad", "frg_get_opts", "frg_autotune", "frg_sort_dedupe_merge_device", "frg_track_batch_noalloc"]
+           "frg_set_step_model", "frg_step_model_eval", "frg_step_model_grad", "frg_get_opts", "frg_autotune", "frg_sort_dedupe_merge_device", "frg_track_batch_noalloc",
+           "frg_step_factor"]
 STEP_MODEL_SIZE, STEP_MODEL_WEIGHTS, STEP_RECORD = 4819, 4801, 9
 
 
@@ -138,6 +139,7 @@ def lib() -> ct.CDLL:
         L.frg_track_batch.argtypes = [P, ct.c_int32, P, P, P, P, P, P, P, P, ct.POINTER(Stats)]
         L.frg_generate_starts.argtypes = [P, ct.c_uint64, ct.c_uint64, ct.c_int64, P, P]
         L.frg_solve_step.argtypes = [P, ct.c_int32, ct.c_int32, ct.c_int64, P, P, P, P, P, C64, P, P]
+        L.frg_step_factor.argtypes = [P, ct.c_int64, P, P]
         L.frg_sort_dedupe_merge.argtypes = [P, ct.c_int64, P, P, ct.c_int64, ct.c_double, P]
         L.frg_sort_dedupe_merge.restype = ct.c_int64
         L.frg_lambertw.argtypes = [ct.c_double]
@@ -353,6 +355,13 @@ class Tracker:
         spm = np.zeros((n, NP), np.complex128)
         _check(lib().frg_generate_starts(self._h, seed, first_path_id, n, _p(spt), _p(spm)), "frg_generate_starts")
         return spt, spm
+
+    def step_factor(self, err):
+        """The tracker's RK step-size factor for the error estimates `err` (frg_step_factor)."""
+        e = np.ascontiguousarray(err, np.float64)
+        out = np.zeros(e.shape[0], np.float64)
+        _check(lib().frg_step_factor(self._h, e.shape[0], _p(e), _p(out)), "frg_step_factor")
+        return out
 
     def solve_step(self, mode: int, Y, u, w, spm, fpm, gamma, tgoal=None, precision: int = PREC_FP64):
         Y = _c(Y)

@@ -752,6 +752,19 @@ int frg_solve_step(frg_ctx *c, int32_t mode, int32_t precision, int64_t n, const
     return FRG_OK;
 }
 
+int frg_step_factor(frg_ctx *c, int64_t n, const double *err, double *scale) {
+    if (!c || n < 0 || (n > 0 && (!err || !scale))) return FRG_EINVAL;
+    if (n == 0) return FRG_OK;
+    BEGIN(c);
+    HIPCHK(c->dbg_e.ensure(sizeof(double) * n));
+    HIPCHK(c->dbg_x.ensure(sizeof(double) * n));
+    HIPCHK(hipMemcpyAsync(c->dbg_e.p, err, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(frg::launch_debug_step_factor(c->dbg_e.as<double>(), n, c->opts.rk_tol, c->dbg_x.as<double>(), c->stream));
+    HIPCHK(hipMemcpyAsync(scale, c->dbg_x.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FRG_OK;
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------------------------------------------

@@ -89,7 +89,10 @@ __host__ __device__ __forceinline__ cx<R> sel(bool c, cx<R> a, cx<R> b) { return
 // Double-double arithmetic (value = hi + lo, |lo| <= ulp(hi)/2): Knuth's TwoSum, Dekker's
 // product through a fused multiply-add, and the accurate sum and product of Hida, Li and Bailey
 // ("Algorithms for quad-double precision floating point arithmetic", 2001).  About 106 bits;
-// used for the residual of the final root refinement only.
+// used for the residual of the final root refinement and for pow_sixth.  Every function carries
+// `#pragma clang fp contract(off)`: the transformations are exact only if each product and sum
+// is rounded by itself, and hipcc fuses a product with a later sum into an FMA by default (after
+// inlining, across these functions), which turns TwoSum and QuickTwoSum into plain sums.
 // ------------------------------------------------------------------------------------------
 struct dd {
     double hi, lo;
@@ -98,34 +101,75 @@ struct dd {
     __host__ __device__ dd(double a, double b) : hi(a), lo(b) {}
 };
 __host__ __device__ __forceinline__ dd two_sum(double a, double b) {
+#pragma clang fp contract(off)
     const double s = a + b, bb = s - a;
     return dd(s, (a - (s - bb)) + (b - bb));
 }
 __host__ __device__ __forceinline__ dd quick_two_sum(double a, double b) {  // |a| >= |b|
+#pragma clang fp contract(off)
     const double s = a + b;
     return dd(s, b - (s - a));
 }
 __host__ __device__ __forceinline__ dd two_prod(double a, double b) {
+#pragma clang fp contract(off)
     const double p = a * b;
     return dd(p, __builtin_fma(a, b, -p));
 }
 __host__ __device__ __forceinline__ dd operator+(dd a, dd b) {
+#pragma clang fp contract(off)
     dd s = two_sum(a.hi, b.hi);
     const dd t = two_sum(a.lo, b.lo);
     s = quick_two_sum(s.hi, s.lo + t.hi);
     return quick_two_sum(s.hi, s.lo + t.lo);
 }
 __host__ __device__ __forceinline__ dd operator-(dd a) { return dd(-a.hi, -a.lo); }
-__host__ __device__ __forceinline__ dd operator-(dd a, dd b) { return a + (-b); }
+__host__ __device__ __forceinline__ dd operator-(dd a, dd b) {
+#pragma clang fp contract(off)
+    return a + (-b);
+}
 __host__ __device__ __forceinline__ dd operator*(dd a, dd b) {
+#pragma clang fp contract(off)
     const dd p = two_prod(a.hi, b.hi);
     return quick_two_sum(p.hi, p.lo + (a.hi * b.lo + a.lo * b.hi));
 }
 // 1/a: a double estimate x, then one Newton step x + x (1 - a x) in double-double
 __host__ __device__ __forceinline__ dd recip(dd a) {
+#pragma clang fp contract(off)
     const double x = 1.0 / a.hi;
     const dd e = dd(1.0) - a * dd(x);
     return dd(x) + dd(x) * e;
+}
+
+// pow(x, 1./6.), correctly rounded, for the RK step-size factor (rungekutta.cu:107-112 compute
+// pow((tol/err), 1./6.); the exponent is the double nearest 1/6, 1/6 - kSixthDelta).
+//   x = y 2^(6k) with y in [0.5, 32), so x^(1/6) = 2^k y^(1/6);
+//   r: y^(1/6) from the hardware log2 and exp2 in single precision (relative error below 1e-6),
+//      then one Newton step on r^6 = y in FP64 (relative error below 1e-11);
+//   c = y / r^6 - 1 with r^6 in double-double (error-free products), so that
+//      y^(1/6) = r (1 + c)^(1/6) = r (1 + c/6 - 5c^2/72 + ...) to far below an ulp;
+//   x^(1/6 - kSixthDelta) = x^(1/6) (1 - kSixthDelta ln x + ...), ln x from the same log2;
+//   one rounding: r + r t with t = c/6 - 5c^2/72 - kSixthDelta ln x, then the exact 2^k.
+// The result is the double nearest x^(1./6.) unless that value lies within about 1e-20 relative
+// of a rounding boundary.  The library pow gets there through a long double-double sequence whose
+// coefficients do not stay in registers inside the tracker loop.  x = 0 gives 0, +inf gives +inf,
+// NaN gives NaN.
+constexpr double kSixthDelta = 9.25185853854297e-18;  // 1/6 - (double)(1./6.)
+__device__ __forceinline__ double pow_sixth(double x) {
+    if (!(x > 0.0) || x == __builtin_inf()) return x == 0.0 ? 0.0 : x;
+    const int e = __builtin_amdgcn_frexp_exp(x);
+    const int k = e >= 0 ? e / 6 : -((5 - e) / 6);
+    const double y = __builtin_amdgcn_ldexp(__builtin_amdgcn_frexp_mant(x), e - 6 * k);
+    const float ly = __builtin_amdgcn_logf((float)y);
+    double r = (double)__builtin_amdgcn_exp2f(ly * (1.0f / 6.0f));
+    const double r5 = (r * r) * (r * r) * r;
+    r = (5.0 * r + y * recip(r5)) * (1.0 / 6.0);
+    const dd r1(r);
+    const dd r3 = (r1 * r1) * r1;
+    const dd r6 = r3 * r3;
+    const double c = (dd(y) - r6).hi * recip(y);
+    const double lnx = (double)((ly + (float)(6 * k)) * 0.6931471805599453f);
+    const double t = c * (1.0 / 6.0) - (5.0 / 72.0) * (c * c) - kSixthDelta * lnx;
+    return __builtin_amdgcn_ldexp(__builtin_fma(r, t, r), k);
 }
 
 // Principal square root by the half-angle formulas of classes.cuh:39-45.

@@ -69,6 +69,8 @@ struct DebugStartArgs {
 };
 hipError_t launch_debug_solve(const DebugSolveArgs &a, int precision, int lanes, hipStream_t stream);
 hipError_t launch_debug_start(const DebugStartArgs &a, hipStream_t stream);
+// the tracker's step-size factor for explicit error estimates (n values)
+hipError_t launch_debug_step_factor(const double *err, int64_t n, double tol, double *out, hipStream_t stream);
 
 // host-side launchers (frg_tracker.hip)
 enum LaunchKind : int { LK_STATIC = 0, LK_PERSISTENT = 1 };

@@ -711,6 +711,13 @@ __device__ __forceinline__ void correct(int role, const TrackOpts &o, PathState<
     P.gs->attempt = 0;
     start_newton(o, P, tn, false);
 }
+// Step-size factor of an RK attempt with error estimate err (rungekutta.cu:107-112):
+// 0.8 pow(tol/err, 1./6.), at most 5, and 5 for err <= 1e-16 or NaN.
+__device__ __forceinline__ double step_factor(double err, double tol) {
+    double scale = err > 10e-17 ? 0.8 * pow_sixth(tol / err) : 5.0;
+    if (scale > 5) scale = 5;
+    return scale;
+}
 // end of an RK attempt (:94-123)
 template <class G>
 __device__ __forceinline__ void finish_rk(int role, const TrackOpts &o, PathState<G> &P) {
@@ -730,8 +737,7 @@ __device__ __forceinline__ void finish_rk(int role, const TrackOpts &o, PathStat
         yn[k] = P.Yc[k] + b;
     }
     const double err = grp_norm<G>(lte);
-    double scale = err > 10e-17 ? 0.8 * pow(o.rk_tol / err, 1. / 6.) : 5.0;
-    if (scale > 5) scale = 5;
+    const double scale = step_factor(err, o.rk_tol);
     P.gs->err = err;
     P.gs->scale = scale;
 #pragma unroll
@@ -1581,6 +1587,18 @@ __global__ void reset_counter_kernel(unsigned long long *p) {
 }
 hipError_t launch_reset_counter(unsigned long long *p, hipStream_t stream) {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, p);
+    return hipGetLastError();
+}
+
+__global__ void debug_step_factor_kernel(const double *err, int64_t n, double tol, double *out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = step_factor(err[i], tol);
+}
+hipError_t launch_debug_step_factor(const double *err, int64_t n, double tol, double *out, hipStream_t stream) {
+    const int64_t g = (n + 255) / 256;
+    if (g == 0) return hipSuccess;
+    if (g > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(debug_step_factor_kernel, dim3((unsigned)g), dim3(256), 0, stream, err, n, tol, out);
     return hipGetLastError();
 }
 

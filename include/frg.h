@@ -194,6 +194,11 @@ int frg_solve_step(frg_ctx *ctx, int32_t mode, int32_t precision, int64_t n, con
                    const frg_c64 *w, const frg_c64 *spm, const frg_c64 *fpm, frg_c64 gamma, const double *tgoal,
                    frg_c64 *x);
 
+/* The step-size factor of an RK attempt as the tracker computes it, for explicit error
+ * estimates err[0..n-1]: scale = 0.8 (rk_tol/err)^(1/6), at most 5, and 5 for err <= 1e-16
+ * (rungekutta.cu:107-112) with the context's rk_tol; stage-level parity entry point. */
+int frg_step_factor(frg_ctx *ctx, int64_t n, const double *err, double *scale);
+
 /* Host post-processing (frgsupport.cu).  frg_sort_dedupe_merge replaces SortingDeletingMerging
  * (frgsupport.cu:10-215) plus the device sort of frg.cu:432-435: merges the n new roots into
  * the sorted collection coll (m roots), marks duplicates in err with code 6 (tolerance tol on
