@@ -56,37 +56,49 @@ __device__ __forceinline__ void unpack8(uint4 q, float out[8]) {
     }
 }
 
+// One lane's share of the model: hidden unit l (the lane index) of both layers, i.e. row l of W1
+// and of W2 and the unit's b1, b2 and w3.
+struct MlpLane {
+    uint4 w1;
+    uint4 w2[8];
+    float b1, b2, w3;
+};
+// Loads this lane's weights (FP16, from L2).
+__device__ __forceinline__ MlpLane mlp_lane_weights(const StepModel &M) {
+    const int l = __lane_id();
+    MlpLane w;
+    w.w1 = reinterpret_cast<const uint4 *>(M.w + kMlpW1 + kMlpIn * l)[0];
+    const uint4 *w2 = reinterpret_cast<const uint4 *>(M.w + kMlpW2 + kMlpH * l);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) w.w2[i] = w2[i];
+    w.b1 = __half2float(M.w[kMlpB1 + l]);
+    w.b2 = __half2float(M.w[kMlpB2 + l]);
+    w.w3 = __half2float(M.w[kMlpW3 + l]);
+    return w;
+}
+
 // Wave-wide evaluation for the groups of one wave that have a pending proposal; W = 8 or 16
 // lanes per group.  Lane l owns hidden unit l of both layers, so its weights are one row of W1
-// and one row of W2, read once per call.  Must be called by all 64 lanes with a wave-uniform
-// `lanes` mask whose set bits are lanes W*g of the pending groups g.  Group g has written its
-// normalised features to area0[g * stride + 0..7] (floats in LDS); the same 128-float area then
-// holds that group's first hidden layer (64 FP16 values at float offset +64) and the 64
+// and one row of W2 (`lw`, from mlp_lane_weights).  Must be called by all 64 lanes with a
+// wave-uniform `lanes` mask whose set bits are lanes W*g of the pending groups g.  Group g has
+// written its normalised features to area0[g * stride + 0..7] (floats in LDS); the same 128-float
+// area then holds that group's first hidden layer (64 FP16 values at float offset +64) and the 64
 // output-layer terms (at +0).  `gw` is this lane's group in the wave, `role` its lane in the
 // group.  Returns the predicted log step size in the lanes of each pending group.
 template <int W = 8>
-__device__ __forceinline__ float mlp_eval_wave(const StepModel &M, float *area0, int stride, uint64_t lanes, int gw,
-                                               int role) {
+__device__ __forceinline__ float mlp_eval_wave(const StepModel &M, const MlpLane &lw, float *area0, int stride,
+                                               uint64_t lanes, int gw, int role) {
     constexpr int kShift = W == 16 ? 4 : 3;
     const int l = __lane_id();
-    const uint4 *w1 = reinterpret_cast<const uint4 *>(M.w + kMlpW1 + kMlpIn * l);
-    const uint4 *w2 = reinterpret_cast<const uint4 *>(M.w + kMlpW2 + kMlpH * l);
-    const uint4 q1 = w1[0];
-    uint4 r2[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) r2[i] = w2[i];
-    const float b1 = __half2float(M.w[kMlpB1 + l]);
-    const float b2 = __half2float(M.w[kMlpB2 + l]);
-    const float w3 = __half2float(M.w[kMlpW3 + l]);
     float a1[8];
-    unpack8(q1, a1);
+    unpack8(lw.w1, a1);
     lds_sync();
     // layer 1 for every pending group
     for (uint64_t m = lanes; m; m &= m - 1) {
         float *area = area0 + (int)(__builtin_ctzll(m) >> kShift) * stride;
         const float4 x0 = reinterpret_cast<const float4 *>(area)[0];
         const float4 x1 = reinterpret_cast<const float4 *>(area)[1];
-        float a = b1;
+        float a = lw.b1;
         a = fmaf(a1[0], x0.x, a);
         a = fmaf(a1[1], x0.y, a);
         a = fmaf(a1[2], x0.z, a);
@@ -98,20 +110,21 @@ __device__ __forceinline__ float mlp_eval_wave(const StepModel &M, float *area0,
         reinterpret_cast<__half *>(area + kMlpH)[l] = __float2half(fast_tanh(a));
     }
     lds_sync();
-    // layer 2 and the output-layer terms w3[l] tanh(.) for every pending group
+    // layer 2 and the output-layer terms w3[l] tanh(.) for every pending group; four partial
+    // sums of 16 products each, so that the dependent chain is 8 dot products long, not 32
     for (uint64_t m = lanes; m; m &= m - 1) {
         float *area = area0 + (int)(__builtin_ctzll(m) >> kShift) * stride;
         const uint4 *hv = reinterpret_cast<const uint4 *>(area + kMlpH);
-        float acc = b2;
+        float acc[4] = {lw.b2, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const uint4 wq = r2[i], hq = hv[i];
-            acc = dot2_f16(wq.x, hq.x, acc);
-            acc = dot2_f16(wq.y, hq.y, acc);
-            acc = dot2_f16(wq.z, hq.z, acc);
-            acc = dot2_f16(wq.w, hq.w, acc);
+            const uint4 wq = lw.w2[i], hq = hv[i];
+            acc[0] = dot2_f16(wq.x, hq.x, acc[0]);
+            acc[1] = dot2_f16(wq.y, hq.y, acc[1]);
+            acc[2] = dot2_f16(wq.z, hq.z, acc[2]);
+            acc[3] = dot2_f16(wq.w, hq.w, acc[3]);
         }
-        area[l] = w3 * fast_tanh(acc);
+        area[l] = lw.w3 * fast_tanh((acc[0] + acc[1]) + (acc[2] + acc[3]));
     }
     lds_sync();
     // each group sums its own 64 terms: 64/W per lane, then over the group

@@ -153,7 +153,8 @@ __global__ __launch_bounds__(64) void mlp_eval_kernel(StepModel M, const float *
     const bool valid = i < n;
     if (valid) area[g][role] = (x[i * kMlpIn + role] - M.xmu[role]) / M.xsd[role];
     const uint64_t lanes = __ballot(valid && role == 0);
-    const float out = mlp_eval_wave(M, &area[0][0], 128, lanes, g, role);
+    const MlpLane lw = mlp_lane_weights(M);
+    const float out = mlp_eval_wave(M, lw, &area[0][0], 128, lanes, g, role);
     if (valid && role == 0) y[i] = out;
 }
 

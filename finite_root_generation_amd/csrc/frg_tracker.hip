@@ -1156,7 +1156,8 @@ __device__ __forceinline__ void track_body(const Job &J) {
             const bool pend = live && P.pending;
             const uint64_t lanes = __ballot(pend && role == 0);
             if (lanes) {
-                const float y = mlp_eval_wave<G::kW>(J.model, xarea0, kXStride, lanes, grp % kGroupsPerWave, role);
+                const float y = mlp_eval_wave<G::kW>(J.model, mlp_lane_weights(J.model), xarea0, kXStride, lanes,
+                                                     grp % kGroupsPerWave, role);
                 if (pend) finish_proposal(J.o, P, y);
             }
         }
