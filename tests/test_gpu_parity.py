@@ -190,10 +190,10 @@ def rounding_sensitive(oo, fpm, gamma, spt, spm, seed, pid, trials=256, scales=(
     return False
 
 
-def _compare_tracks(fpt_g, err_g, fpt_o, err_o, min_agree=1.0, explain=None, min_agree_explained=0.98):
+def _compare_tracks(fpt_g, err_g, fpt_o, err_o, min_agree=1.0, explain=None):
     """Per-path agreement of error codes and (finite) roots.  With `explain` = (opts, fpm,
     gamma, spt, spm, seed, pid0), a disagreeing path is accepted only if the oracle shows it to
-    be rounding-sensitive, and at least `min_agree_explained` of the paths must agree."""
+    be rounding-sensitive."""
     same_code = err_g == err_o
     ok = tasks.finite_mask(fpt_o, err_o) & tasks.finite_mask(fpt_g, err_g)
     d = abs_diff(fpt_g, fpt_o)
@@ -212,7 +212,7 @@ def _compare_tracks(fpt_g, err_g, fpt_o, err_o, min_agree=1.0, explain=None, min
     assert np.max(d[ok & agree]) <= ROOT_TOL if (ok & agree).any() else True
     if explain is not None:
         assert set(bad.tolist()) == set(explained), "GPU/oracle disagreement on a rounding-stable path"
-        assert agree.mean() >= min_agree_explained
+        assert agree.mean() >= 0.98
     else:
         assert agree.mean() >= min_agree, bad[:20]
 
@@ -322,11 +322,7 @@ def test_track_real_task_singular_endpoints(tracker):
     oo = O.default_opts()
     fpt_o, err_o, st_o, _ = O.homotopy(oo, fpm, gamma, 7, 0, n)
     spt, spm = _starts(7, n)
-    # 43 % of these paths end at singular roots whose sharpening (code 5) or end point rounding
-    # decides, so a change of rounding anywhere in the tracker moves a few of them either way:
-    # the floor on agreement is lower here, the rule that every disagreement is
-    # rounding-sensitive in the oracle is the same
-    _compare_tracks(fpt_g, err_g, fpt_o, err_o, explain=(oo, fpm, gamma, spt, spm, 7, 0), min_agree_explained=0.95)
+    _compare_tracks(fpt_g, err_g, fpt_o, err_o, explain=(oo, fpm, gamma, spt, spm, 7, 0))
 
 
 def test_persistent_equals_static(tracker):
