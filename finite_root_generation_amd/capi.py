@@ -70,6 +70,8 @@ class Opts(ct.Structure):
         ("polish_its", ct.c_int32),
         ("waves_per_simd", ct.c_int32),
         ("pad0", ct.c_int32),
+        ("ml_safety", ct.c_double),
+        ("ml_growth", ct.c_double),
     ]
 
 

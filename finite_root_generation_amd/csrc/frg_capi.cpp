@@ -123,6 +123,8 @@ frg::TrackOpts to_track_opts(const frg_opts &o) {
     t.start_mode = o.start_mode;
     t.precision = o.precision;
     t.fp64_band = o.fp64_band;
+    t.ml_safety = o.ml_safety;
+    t.ml_growth = o.ml_growth;
     return t;
 }
 
@@ -149,6 +151,7 @@ int check_opts(const frg_opts &o) {
     if (o.waves_per_simd < 0 || o.waves_per_simd > 2) return FRG_EINVAL;
     if (o.waves_per_simd == 2 && o.lanes_per_path != 16) return FRG_EUNSUPPORTED;  // 16-lane kernel only
     if (!(o.smin > 0.0) || !(o.rk_tol > 0.0)) return FRG_EINVAL;
+    if (!(o.ml_safety > 0.0) || !(o.ml_growth >= 1.0) || !(o.ml_safety < 1e6) || !(o.ml_growth < 1e6)) return FRG_EINVAL;
     return FRG_OK;
 }
 
@@ -388,6 +391,8 @@ int frg_default_opts(frg_opts *o) {
     o->start_mode = FRG_START_UNIFORM;
     o->precision = FRG_PREC_FP64;
     o->fp64_band = 1e-2;
+    o->ml_safety = 0.6;
+    o->ml_growth = 5.0;
     o->schedule = FRG_SCHED_STATIC;
     o->block_size = 0;
     o->groups_per_cu = 0;

@@ -24,6 +24,8 @@ struct TrackOpts {
     int start_mode;     // 0: U[start_lo, start_hi]^2 (reference), 1: standard complex Gaussian
     int precision;      // 0: FP64, 1: FP32 Jacobian/solve away from t = 1
     double fp64_band;   // mixed precision: FP64 once 1 - t < fp64_band
+    double ml_safety;   // learned step control: factor on the model's step
+    double ml_growth;   // learned step control: cap on the model's step as a multiple of the current one
 };
 
 // One launch.  Several instances (task targets) may share a launch; instance i owns global path

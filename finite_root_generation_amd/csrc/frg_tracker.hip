@@ -865,11 +865,12 @@ __device__ __forceinline__ void accept_step(int role, const Job &J, PathState<G>
 template <class G>
 __device__ __forceinline__ void finish_proposal(const TrackOpts &o, PathState<G> &P, float y) {
     double s_next = P.s_classic;
-    // safety factor on the model's step: 0.6 minimised RK attempts per path in a sweep over
-    // 0.35..1.15 (2^18 paths of a seeded task): fewer rejections outweigh the shorter steps
-    double s_ml = (double)__expf(y) * 0.6;
+    // safety factor on the model's step (default 0.6: the minimum of RK attempts per path in a
+    // sweep over 0.35..1.15 on 2^18 paths of a seeded task; fewer rejections outweigh the shorter
+    // steps) and cap on its growth (default 5, the classical rule's)
+    double s_ml = (double)__expf(y) * o.ml_safety;
     if (s_ml == s_ml && s_ml < 1e300) {
-        s_ml = fmin(s_ml, 5.0 * P.s);
+        s_ml = fmin(s_ml, o.ml_growth * P.s);
         if (P.gs->streak < o.caution_steps) s_ml = fmin(s_ml, P.s);
         s_next = s_ml;
     }

@@ -79,6 +79,10 @@ typedef struct {
     int32_t waves_per_simd;                /* register budget of the 16-lane tracker as resident waves per SIMD:
                                               1 (up to 512 VGPRs per lane) or 2 (256); 0 = 1.  Only 1 with 8 lanes */
     int32_t pad0;
+    double ml_safety;                      /* learned step control: the model's step is exp(prediction) times this
+                                              factor (default 0.6) */
+    double ml_growth;                      /* learned step control: the model's step is at most this multiple of
+                                              the current step (default 5, the classical rule's cap) */
 } frg_opts;
 
 /* Per-call summary. */
