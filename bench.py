@@ -13,7 +13,7 @@ codes, per-path counters) stay in HBM, so the timed region is the launch sequenc
             seeds: the 4,096-path task 0 and this task 1), launch shape picked by the autotuner
             on a 2^19-path sample;
   * ablations (one launch each): persistent + classical, mixed FP32/FP64 + persistent + classical,
-    and mixed + persistent + learned (the north star's combination on one instance).
+    and mixed + persistent + learned (all three changes together on one instance).
 value = chosen paths/s (wall clock around the timed steps).  One JSON line on stdout.
 
 The line also carries "batched", the many-instance workload: 64 task instances (seeded tasks
@@ -212,7 +212,7 @@ def per_path_vs_oracle(tasks, g_fpt, g_err, o_fpt, o_err):
 
 
 def train_step_model(capi, tasks):
-    """Step logs that the tracker writes on the configs' own tasks (task 0 of the 4,096-path
+    """Step logs that the tracker writes on the workloads' own tasks (task 0 of the 4,096-path
     workload and task 1 of this one) with start seeds other than the measured one; one model
     trained on the GPU."""
     t0 = time.perf_counter()

@@ -959,8 +959,8 @@ void oracle_cognate_solns(const ocplx *z, const ocplx *q, ocplx *out) {
  * ends: the point is not in Newton's quadratic basin. */
 #define POLISH_CAP 1e-6
 
-/* Final FP64 refinement of a code-0 root (not in the reference; the north star's "final root
- * refinement"): polish_its Newton steps at t = 1 in the dehomogenising patch e_21 that
+/* Final FP64 refinement of a code-0 root (not in the reference; the final root refinement of the
+ * tracker's output): polish_its Newton steps at t = 1 in the dehomogenising patch e_21 that
  * SwitchPatch has just applied (frg.cu:44-46,60), with Z0 = 1 exactly, so the root is refined in
  * the coordinates it is reported in.  Each step is iterative refinement with an extended
  * residual: the Jacobian, the row scaling and the solve are the double-precision Newton step of

@@ -1,6 +1,6 @@
 """The batched and the learned workloads on their own inputs.
 
-Config 4 (many instances in one launch): 64 task instances (seeded tasks 500..563, start seeds
+Many instances in one launch: 64 task instances (seeded tasks 500..563, start seeds
 600..663, Gaussian start draws) with path counts log-uniform in [64, 16,384] (the workload's
 [1,024, 262,144] scaled by 1/16 to fit the test budget), FP32/FP64 mixed precision and learned
 step control, launch shape picked by the autotuner, all instances in one persistent launch
@@ -9,7 +9,7 @@ finite roots of separate plain FP64 launches (static, classical, one per instanc
 1e-8; the FP64 classical variant of the same batch equals the separate launches bit for bit; and
 the oracle agrees path by path with the plain launches on a sample of instances.
 
-Config 5 (learned step size): the model is trained on 10M step records that the tracker writes on
+Learned step size: the model is trained on 10M step records that the tracker writes on
 the tasks of the 4,096-path and full-size workloads (tasks 0 and 1, start seeds other than the
 measured ones), then run on the 4,096-path workload's own inputs (task 0, start seed 0), in FP64
 and in mixed precision, against the classical controller and the oracle's FP64 roots."""
@@ -33,8 +33,9 @@ def say(msg):
 
 
 @pytest.fixture(scope="module")
-def model45():
-    """Step model trained on 10M records of the configs' tasks (tasks 0 and 1, start seeds 100+)."""
+def step_model():
+    """Step model trained on 10M records of the 4,096-path and full-size tasks (tasks 0 and 1,
+    start seeds 100+)."""
     recs = []
     with capi.Tracker(0, start_mode=capi.START_GAUSSIAN, schedule=capi.SCHED_PERSISTENT) as t:
         for task, seed in ((0, 100), (1, 101), (0, 102), (1, 103)):
@@ -56,7 +57,7 @@ def _recall(ref_roots, roots):
     return float(np.mean([found_anywhere(r, x) for x in ref_roots]))
 
 
-def test_config4_instances_in_one_graph_launch(model45):
+def test_64_instances_in_one_graph_launch(step_model):
     rng = np.random.default_rng(4)
     k_inst = 64
     counts = np.exp(rng.uniform(np.log(64), np.log(16384), k_inst)).astype(np.int64)
@@ -68,11 +69,11 @@ def test_config4_instances_in_one_graph_launch(model45):
     n = int(offs[-1])
     with capi.Tracker(0, precision=capi.PREC_MIXED, use_graph=1, step_control=capi.STEP_LEARNED,
                       start_mode=capi.START_GAUSSIAN) as t:
-        t.set_step_model(model45)
-        say(f"[gpu] config 4: autotune on 32768 paths")
+        t.set_step_model(step_model)
+        say(f"[gpu] 64 instances: autotune on 32768 paths")
         rows, chosen = t.autotune(fpms[0], gams[0], seed=9, n_sample=32768, first_path_id=1 << 40)
         say(f"    chosen schedule/block/groups-per-CU {chosen} of {len(rows)} candidates")
-        say(f"[gpu] config 4: {k_inst} instances, {n} paths, one graph launch (and a replay)")
+        say(f"[gpu] many instances: {k_inst} instances, {n} paths, one graph launch (and a replay)")
         fpt, err, st, summ = t.track_batch(fpms, gams, seeds, pids, counts)
         fpt2, err2, _, summ2 = t.track_batch(fpms, gams, seeds, pids, counts)
     ok = tasks.finite_mask(fpt, err)
@@ -132,15 +133,15 @@ def test_config4_instances_in_one_graph_launch(model45):
 
 
 @pytest.mark.parametrize("precision", [capi.PREC_FP64, capi.PREC_MIXED], ids=["fp64", "mixed"])
-def test_config5_learned_on_the_4096_path_inputs(model45, precision):
+def test_learned_on_the_4096_path_inputs(step_model, precision):
     fpm, gamma = tasks.seeded_task(0)
     n = 4096
     res = {}
     for name, sc in (("classical", capi.STEP_CLASSIC), ("learned", capi.STEP_LEARNED)):
         with capi.Tracker(0, precision=precision, step_control=sc, start_mode=capi.START_GAUSSIAN) as t:
             if sc == capi.STEP_LEARNED:
-                t.set_step_model(model45)
-            say(f"[gpu] config 5 ({'fp64' if precision == 0 else 'mixed'}): {name}, {n} paths")
+                t.set_step_model(step_model)
+            say(f"[gpu] learned, 4096 paths ({'fp64' if precision == 0 else 'mixed'}): {name}, {n} paths")
             res[name] = t.track(fpm, gamma, seed=0, n=n)
     fo, eo, _, _ = O.homotopy(O.default_opts(start_mode=capi.START_GAUSSIAN), fpm, gamma, 0, 0, n)
     oracle_roots = fo[tasks.finite_mask(fo, eo)]
