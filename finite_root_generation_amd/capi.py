@@ -386,7 +386,7 @@ def sort_dedupe_merge(coll, fpt, err, tol: float = 1e-6):
     out = np.zeros((coll.shape[0] + fpt.shape[0], NV), np.complex128)
     k = lib().frg_sort_dedupe_merge(_p(coll), coll.shape[0], _p(fpt), _p(err), fpt.shape[0], float(tol), _p(out))
     if k < 0:
-        raise FrgError("frg_sort_dedupe_merge: invalid argument")
+        _check(int(k), "frg_sort_dedupe_merge")
     return out[:k], err
 
 

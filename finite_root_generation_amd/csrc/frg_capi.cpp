@@ -77,6 +77,10 @@ struct frg_ctx {
 
 namespace {
 
+// Largest path or record count that one call accepts (FRG_MAX_COUNT = 2^36): beyond any GPU's
+// memory (2^36 roots are 23 TB), and small enough that no byte count computed from it overflows.
+bool bad_count(int64_t n) { return n < 0 || n > FRG_MAX_COUNT; }
+
 #define HIPCHK(x)                              \
     do {                                       \
         hipError_t e_ = (x);                   \
@@ -271,7 +275,7 @@ int upload_instances(frg_ctx *c, int32_t n_inst, const frg_c64 *fpm, const frg_c
                      const uint64_t *pid0s, const int64_t *counts, frg::Job &job, hipStream_t stream) {
     std::vector<int64_t> first(n_inst + 1, 0);
     for (int i = 0; i < n_inst; ++i) {
-        if (counts[i] < 0) return FRG_EINVAL;
+        if (bad_count(counts[i]) || first[i] + counts[i] > FRG_MAX_COUNT) return FRG_EINVAL;
         first[i + 1] = first[i] + counts[i];
     }
     HIPCHK(c->inst_first.ensure(sizeof(int64_t) * (n_inst + 1)));
@@ -477,19 +481,21 @@ int frg_get_opts(const frg_ctx *c, frg_opts *o) {
 }
 
 int frg_track(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id, int64_t n_paths,
-              frg_c64 *fpt, int32_t *err, int32_t *path_stats, frg_stats *summary) {
-    if (!c || !fpm || n_paths < 0 || (n_paths > 0 && (!fpt || !err))) return FRG_EINVAL;
+              frg_c64 *fpt, int32_t *err, int32_t *path_stats, frg_stats *summary) try {
+    if (!c || !fpm || bad_count(n_paths) || (n_paths > 0 && (!fpt || !err))) return FRG_EINVAL;
     auto t0 = std::chrono::steady_clock::now();
     BEGIN(c);
     frg::Job job{};
     int rc = upload_instances(c, 1, fpm, &gamma, &seed, &first_path_id, &n_paths, job, c->stream);
     if (rc != FRG_OK) return rc;
     return run_host(c, job, fpt, err, path_stats, nullptr, summary, t0);
+} catch (const std::bad_alloc &) {
+    return FRG_ENOMEM;
 }
 
 int frg_track_device(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
-                     int64_t n_paths, frg_c64 *d_fpt, int32_t *d_err, int32_t *d_path_stats, void *stream) {
-    if (!c || !fpm || n_paths < 0 || (n_paths > 0 && (!d_fpt || !d_err))) return FRG_EINVAL;
+                     int64_t n_paths, frg_c64 *d_fpt, int32_t *d_err, int32_t *d_path_stats, void *stream) try {
+    if (!c || !fpm || bad_count(n_paths) || (n_paths > 0 && (!d_fpt || !d_err))) return FRG_EINVAL;
     BEGIN(c);
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
     frg::Job job{};
@@ -503,12 +509,14 @@ int frg_track_device(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t see
     if (rc != FRG_OK) return rc;
     HIPCHK(hipEventRecord(c->ev_busy, s));
     return FRG_OK;
+} catch (const std::bad_alloc &) {
+    return FRG_ENOMEM;
 }
 
 int frg_track_explicit(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, const frg_c64 *spt, const frg_c64 *spm,
                        uint64_t seed, uint64_t first_path_id, int64_t n_paths, frg_c64 *fpt, int32_t *err,
-                       int32_t *path_stats, frg_c64 *yfinal, frg_stats *summary) {
-    if (!c || !fpm || n_paths < 0 || (n_paths > 0 && (!spt || !spm || !fpt || !err))) return FRG_EINVAL;
+                       int32_t *path_stats, frg_c64 *yfinal, frg_stats *summary) try {
+    if (!c || !fpm || bad_count(n_paths) || (n_paths > 0 && (!spt || !spm || !fpt || !err))) return FRG_EINVAL;
     auto t0 = std::chrono::steady_clock::now();
     BEGIN(c);
     frg::Job job{};
@@ -523,11 +531,13 @@ int frg_track_explicit(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, const frg_
         job.spm = c->spm.as<cx<double>>();
     }
     return run_host(c, job, fpt, err, path_stats, yfinal, summary, t0);
+} catch (const std::bad_alloc &) {
+    return FRG_ENOMEM;
 }
 
 int frg_track_batch(frg_ctx *c, int32_t n_inst, const frg_c64 *fpm, const frg_c64 *gamma, const uint64_t *seeds,
                     const uint64_t *first_path_ids, const int64_t *paths_per_inst, frg_c64 *fpt, int32_t *err,
-                    int32_t *path_stats, frg_stats *summary) {
+                    int32_t *path_stats, frg_stats *summary) try {
     if (!c || n_inst <= 0 || !fpm || !gamma || !seeds || !first_path_ids || !paths_per_inst) return FRG_EINVAL;
     auto t0 = std::chrono::steady_clock::now();
     BEGIN(c);
@@ -536,6 +546,8 @@ int frg_track_batch(frg_ctx *c, int32_t n_inst, const frg_c64 *fpm, const frg_c6
     if (rc != FRG_OK) return rc;
     if (job.n_total > 0 && (!fpt || !err)) return FRG_EINVAL;
     return run_host(c, job, fpt, err, path_stats, nullptr, summary, t0);
+} catch (const std::bad_alloc &) {
+    return FRG_ENOMEM;
 }
 
 int frg_track_batch_noalloc(frg_ctx *c, int32_t n_inst, const frg_c64 *d_fpm, const frg_c64 *gamma,
@@ -548,7 +560,7 @@ int frg_track_batch_noalloc(frg_ctx *c, int32_t n_inst, const frg_c64 *d_fpm, co
     st.n_inst = n_inst;
     st.inst_first[0] = 0;
     for (int i = 0; i < n_inst; ++i) {
-        if (paths_per_inst[i] < 0) return FRG_EINVAL;
+        if (bad_count(paths_per_inst[i]) || st.inst_first[i] + paths_per_inst[i] > FRG_MAX_COUNT) return FRG_EINVAL;
         st.inst_first[i + 1] = st.inst_first[i] + paths_per_inst[i];
         st.gamma[i] = frg::mk(gamma[i].re, gamma[i].im);
         st.seed[i] = seeds[i];
@@ -580,8 +592,9 @@ int frg_track_batch_noalloc(frg_ctx *c, int32_t n_inst, const frg_c64 *d_fpm, co
 }
 
 int frg_autotune(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
-                 int64_t n_sample, frg_tune_entry *table, int32_t table_cap, int32_t *n_entries) {
-    if (!c || !fpm || n_sample <= 0 || table_cap < 0 || (table_cap > 0 && !table)) return FRG_EINVAL;
+                 int64_t n_sample, frg_tune_entry *table, int32_t table_cap, int32_t *n_entries) try {
+    if (!c || !fpm || n_sample <= 0 || bad_count(n_sample) || table_cap < 0 || (table_cap > 0 && !table))
+        return FRG_EINVAL;
     BEGIN(c);
     const frg_opts saved = c->opts;
     frg::Job job{};
@@ -655,12 +668,14 @@ int frg_autotune(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, u
     drop_graph(c);
     if (n_entries) *n_entries = k;
     return FRG_OK;
+} catch (const std::bad_alloc &) {
+    return FRG_ENOMEM;
 }
 
 int frg_sort_dedupe_merge_device(frg_ctx *c, const frg_c64 *d_coll, int64_t m, const frg_c64 *d_fpt,
                                  int32_t *d_err, int64_t n, double tol, frg_c64 *d_coll_out, int64_t *m_out,
                                  void *stream) {
-    if (!c || m < 0 || n < 0 || (m > 0 && !d_coll) || (n > 0 && (!d_fpt || !d_err)) || !d_coll_out || !m_out ||
+    if (!c || bad_count(m) || bad_count(n) || (m > 0 && !d_coll) || (n > 0 && (!d_fpt || !d_err)) || !d_coll_out || !m_out ||
         !(tol >= 0.0) || m + n >= ((int64_t)1 << 31))
         return FRG_EINVAL;
     BEGIN(c);
@@ -696,7 +711,7 @@ int frg_last_refine_ms(frg_ctx *c, double *ms) {
 }
 
 int frg_generate_starts(frg_ctx *c, uint64_t seed, uint64_t first_path_id, int64_t n, frg_c64 *spt, frg_c64 *spm) {
-    if (!c || n < 0 || (n > 0 && (!spt || !spm))) return FRG_EINVAL;
+    if (!c || bad_count(n) || (n > 0 && (!spt || !spm))) return FRG_EINVAL;
     if (n == 0) return FRG_OK;
     BEGIN(c);
     HIPCHK(c->spt.ensure(sizeof(frg_c64) * 21 * n));
@@ -718,7 +733,7 @@ int frg_generate_starts(frg_ctx *c, uint64_t seed, uint64_t first_path_id, int64
 int frg_solve_step(frg_ctx *c, int32_t mode, int32_t precision, int64_t n, const frg_c64 *Y, const frg_c64 *u,
                    const frg_c64 *w, const frg_c64 *spm, const frg_c64 *fpm, frg_c64 gamma, const double *tgoal,
                    frg_c64 *x) {
-    if (!c || n < 0 || mode < 0 || mode > 2) return FRG_EINVAL;
+    if (!c || bad_count(n) || mode < 0 || mode > 2) return FRG_EINVAL;
     if (precision != FRG_PREC_FP64 && precision != FRG_PREC_MIXED) return FRG_EINVAL;
     if (pick_lanes(c) == 16 && precision != FRG_PREC_FP64) return FRG_EUNSUPPORTED;
     if (n == 0) return FRG_OK;
@@ -758,7 +773,7 @@ int frg_solve_step(frg_ctx *c, int32_t mode, int32_t precision, int64_t n, const
 }
 
 int frg_step_factor(frg_ctx *c, int64_t n, const double *err, double *scale) {
-    if (!c || n < 0 || (n > 0 && (!err || !scale))) return FRG_EINVAL;
+    if (!c || bad_count(n) || (n > 0 && (!err || !scale))) return FRG_EINVAL;
     if (n == 0) return FRG_OK;
     BEGIN(c);
     HIPCHK(c->dbg_e.ensure(sizeof(double) * n));
@@ -809,8 +824,8 @@ void record_stats(const float *rec, int64_t n, float *norm) {
 extern "C" {
 
 int frg_collect_step_log(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t seed, uint64_t first_path_id,
-                         int64_t n_paths, float *records, int64_t cap, int64_t *n_records) {
-    if (!c || !fpm || n_paths < 0 || cap < 0 || cap > INT32_MAX || (cap > 0 && !records) || !n_records)
+                         int64_t n_paths, float *records, int64_t cap, int64_t *n_records) try {
+    if (!c || !fpm || bad_count(n_paths) || cap < 0 || cap > INT32_MAX || (cap > 0 && !records) || !n_records)
         return FRG_EINVAL;
     BEGIN(c);
     frg::Job job{};
@@ -845,6 +860,8 @@ int frg_collect_step_log(frg_ctx *c, const frg_c64 *fpm, frg_c64 gamma, uint64_t
     }
     *n_records = kept;
     return FRG_OK;
+} catch (const std::bad_alloc &) {
+    return FRG_ENOMEM;
 }
 
 int frg_set_step_model(frg_ctx *c, const float *model) {
@@ -867,7 +884,7 @@ int frg_set_step_model(frg_ctx *c, const float *model) {
 }
 
 int frg_step_model_eval(frg_ctx *c, const float *features, int64_t n, float *out) {
-    if (!c || n < 0 || (n > 0 && (!features || !out)) || !c->have_model) return FRG_EINVAL;
+    if (!c || bad_count(n) || (n > 0 && (!features || !out)) || !c->have_model) return FRG_EINVAL;
     if (n == 0) return FRG_OK;
     BEGIN(c);
     HIPCHK(c->tsamples.ensure(sizeof(float) * frg::kMlpIn * n));
@@ -880,8 +897,9 @@ int frg_step_model_eval(frg_ctx *c, const float *features, int64_t n, float *out
     return FRG_OK;
 }
 
-int frg_step_model_grad(frg_ctx *c, const float *model, const float *records, int64_t n, float *grad, float *loss) {
-    if (!c || !model || !records || n <= 0 || !grad) return FRG_EINVAL;
+int frg_step_model_grad(frg_ctx *c, const float *model, const float *records, int64_t n, float *grad,
+                        float *loss) try {
+    if (!c || !model || !records || n <= 0 || bad_count(n) || !grad) return FRG_EINVAL;
     BEGIN(c);
     HIPCHK(c->tw.ensure(sizeof(float) * frg::kMlpNW));
     HIPCHK(c->tgrad.ensure(sizeof(float) * frg::kMlpNW));
@@ -914,11 +932,14 @@ int frg_step_model_grad(frg_ctx *c, const float *model, const float *records, in
     HIPCHK(hipStreamSynchronize(c->stream));
     if (loss) *loss = l;
     return FRG_OK;
+} catch (const std::bad_alloc &) {
+    return FRG_ENOMEM;
 }
 
 int frg_train_step_model(frg_ctx *c, const float *records, int64_t n, int32_t steps, int32_t batch, float lr,
-                         uint64_t seed, float *model, float *loss) {
-    if (!c || !records || n <= 0 || steps < 0 || batch <= 0 || !(lr > 0.f) || !model) return FRG_EINVAL;
+                         uint64_t seed, float *model, float *loss) try {
+    if (!c || !records || n <= 0 || bad_count(n) || steps < 0 || batch <= 0 || !(lr > 0.f) || !model)
+        return FRG_EINVAL;
     BEGIN(c);
     // normalisation from the data; Glorot-uniform initial weights from a host generator
     std::vector<float> h((size_t)FRG_STEP_MODEL_SIZE, 0.f);
@@ -989,6 +1010,8 @@ int frg_train_step_model(frg_ctx *c, const float *records, int64_t n, int32_t st
     HIPCHK(hipStreamSynchronize(c->stream));
     std::memcpy(model, h.data(), sizeof(float) * FRG_STEP_MODEL_SIZE);
     return FRG_OK;
+} catch (const std::bad_alloc &) {
+    return FRG_ENOMEM;
 }
 
 }  // extern "C"

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <new>
 #include <numeric>
 #include <vector>
 
@@ -68,8 +69,10 @@ double frg_completeness(int64_t successes, int64_t duplicates) {
 }
 
 int64_t frg_sort_dedupe_merge(const frg_c64 *coll, int64_t m, const frg_c64 *fpt, int32_t *err, int64_t n, double tol,
-                              frg_c64 *coll_out) {
-    if (m < 0 || n < 0 || (m > 0 && !coll) || (n > 0 && (!fpt || !err)) || !coll_out || !(tol >= 0.0)) return -1;
+                              frg_c64 *coll_out) try {
+    if (m < 0 || n < 0 || m > FRG_MAX_COUNT || n > FRG_MAX_COUNT || (m > 0 && !coll) || (n > 0 && (!fpt || !err)) ||
+        !coll_out || !(tol >= 0.0))
+        return FRG_EINVAL;
     auto key = [&](int64_t g) -> double { return g < m ? coll[g * FRG_NV].re : fpt[(g - m) * FRG_NV].re; };
     // new roots in ascending Re z_0 (stable; NaN keys last)
     std::vector<int64_t> ord((size_t)n);
@@ -116,6 +119,8 @@ int64_t frg_sort_dedupe_merge(const frg_c64 *coll, int64_t m, const frg_c64 *fpt
         if (src) std::memcpy(coll_out + (k++) * FRG_NV, src, sizeof(frg_c64) * FRG_NV);
     }
     return k;
+} catch (const std::bad_alloc &) {
+    return FRG_ENOMEM;
 }
 
 }  // extern "C"

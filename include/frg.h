@@ -12,7 +12,8 @@
  * (htpytrack.cu:85-216).  No HIP or torch types cross this boundary.
  *
  * Conventions: functions return FRG_OK (0) or a negative FRG_E* code; per-path outcomes are in
- * the err[] output.  The caller owns every host buffer; a context owns its device memory and
+ * the err[] output.  A call takes at most FRG_MAX_COUNT paths, roots or records (FRG_EINVAL
+ * beyond); a failed host allocation returns FRG_ENOMEM, and no C++ exception leaves the library.  The caller owns every host buffer; a context owns its device memory and
  * stream.  A context is used by one host thread at a time; separate contexts are independent.
  * Work that an entry point leaves running on a caller's stream (frg_track_device,
  * frg_sort_dedupe_merge_device) uses the context's buffers: the next call on the same context,
@@ -31,6 +32,7 @@ extern "C" {
 #define FRG_NV 21 /* variables, htpykinematics.cuh:8 */
 #define FRG_NP 30 /* parameters, htpykinematics.cuh:9 */
 #define FRG_NY 23 /* homogeneous path point [z, Z0, t] */
+#define FRG_MAX_COUNT ((int64_t)1 << 36) /* paths, roots or records per call */
 
 enum {
     FRG_OK = 0,

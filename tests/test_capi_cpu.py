@@ -47,6 +47,21 @@ def test_strerror():
     assert capi.lib().frg_strerror(-4) == b"unsupported option"
 
 
+def test_counts_beyond_the_per_call_limit_are_rejected():
+    """A path or root count above FRG_MAX_COUNT (2^36) is refused before anything is allocated;
+    a null context fails the same way for every entry point that takes one."""
+    L = capi.lib()
+    big = (1 << 36) + 1
+    one = np.zeros((1, 21), np.complex128)
+    err = np.zeros(1, np.int32)
+    out = np.zeros((2, 21), np.complex128)
+    assert L.frg_sort_dedupe_merge(capi._p(one), big, capi._p(one), capi._p(err), 1, 1e-6, capi._p(out)) == -1
+    assert L.frg_sort_dedupe_merge(capi._p(one), 1, capi._p(one), capi._p(err), big, 1e-6, capi._p(out)) == -1
+    other = np.ones((1, 21), np.complex128)
+    assert L.frg_sort_dedupe_merge(capi._p(one), 1, capi._p(other), capi._p(err), 1, 1e-6, capi._p(out)) == 2
+    assert L.frg_step_factor(None, 1, capi._p(np.ones(1)), capi._p(np.ones(1))) == -1
+
+
 def _roots(rng, n, base):
     picks = rng.integers(0, base.shape[0], n)
     return base[picks] + 1e-9 * (rng.normal(size=(n, 21)) + 1j * rng.normal(size=(n, 21)))

@@ -388,6 +388,23 @@ def test_empty_and_tiny_launches(tracker):
     fpt, err, st, summ = tracker.track(fpm, gamma, seed=1, n=1)
     fo, eo, so, _ = O.homotopy(O.default_opts(), fpm, gamma, 1, 0, 1)
     assert err[0] == eo[0]
+    # a path count above FRG_MAX_COUNT (2^36) is refused before any allocation or launch
+    say("[gpu] n above the per-call limit")
+    L = capi.lib()
+    big = (1 << 36) + 1
+    one = np.zeros((1, 21), np.complex128)
+    e1 = np.zeros(1, np.int32)
+    fpm_c = np.ascontiguousarray(fpm, np.complex128)
+    rc = L.frg_track(tracker._h, capi._p(fpm_c), capi._c64(gamma), 1, 0, big, capi._p(one), capi._p(e1), None, None)
+    assert rc == -1
+    counts = np.array([big], np.int64)
+    g1 = np.array([gamma], np.complex128)
+    u1 = np.ones(1, np.uint64)
+    rc = L.frg_track_batch(tracker._h, 1, capi._p(fpm_c), capi._p(g1), capi._p(u1), capi._p(u1), capi._p(counts),
+                           capi._p(one), capi._p(e1), None, None)
+    assert rc == -1
+    fpt, err, st, summ = tracker.track(fpm, gamma, seed=1, n=1)
+    assert err[0] == eo[0]
 
 
 def test_yfinal_lands_on_target(tracker):
