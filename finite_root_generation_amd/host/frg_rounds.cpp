@@ -175,7 +175,10 @@ int main(int argc, char **argv) {
             }
             coll.resize((size_t)(bytes / sizeof(frg_c64)));
             f.seekg(0);
-            f.read(reinterpret_cast<char *>(coll.data()), bytes);
+            if (!f.read(reinterpret_cast<char *>(coll.data()), bytes)) {
+                std::cerr << "fptColl: read failed" << std::endl;
+                return 1;
+            }
             std::cout << "Previous roots collected: " << coll.size() / FRG_NV << std::endl;
         }
     }
@@ -188,7 +191,10 @@ int main(int argc, char **argv) {
             trials = bytes / (int64_t)sizeof(int32_t);
             std::vector<int32_t> codes((size_t)trials);
             f.seekg(0);
-            f.read(reinterpret_cast<char *>(codes.data()), trials * (int64_t)sizeof(int32_t));
+            if (!f.read(reinterpret_cast<char *>(codes.data()), trials * (int64_t)sizeof(int32_t))) {
+                std::cerr << "errorCodesAgr: read failed" << std::endl;
+                return 1;
+            }
             total.add(codes.data(), trials);
             round = trials / a.paths;
             std::cout << "Previous trials: " << trials << std::endl;
@@ -288,15 +294,30 @@ int main(int argc, char **argv) {
         const double est = frg_completeness(total.code[0], total.code[6]);
         const double t_dedupe = seconds_since(t0);
 
-        // files (frg.cu:472-480)
+        // files (frg.cu:472-480).  The collection is written to a temporary file that then
+        // replaces fptColl, so a failed write (a full disk) leaves the previous collection whole;
+        // either failure ends the run.
         t0 = std::chrono::steady_clock::now();
         {
-            std::ofstream f(path_in(a, "fptColl"), std::ios::binary | std::ios::trunc);
+            const std::string tmp = path_in(a, "fptColl.tmp");
+            std::ofstream f(tmp, std::ios::binary | std::ios::trunc);
             f.write(reinterpret_cast<const char *>(coll.data()), (std::streamsize)(coll.size() * sizeof(frg_c64)));
+            f.close();
+            if (!f || std::rename(tmp.c_str(), path_in(a, "fptColl").c_str()) != 0) {
+                std::cerr << "cannot write fptColl in " << a.dir << std::endl;
+                frg_destroy(ctx);
+                return 1;
+            }
         }
         {
             std::ofstream f(path_in(a, "errorCodesAgr"), std::ios::binary | std::ios::app);
             f.write(reinterpret_cast<const char *>(err), (std::streamsize)(n * sizeof(int32_t)));
+            f.close();
+            if (!f) {
+                std::cerr << "cannot append to errorCodesAgr in " << a.dir << std::endl;
+                frg_destroy(ctx);
+                return 1;
+            }
         }
         const double t_write = seconds_since(t0);
 

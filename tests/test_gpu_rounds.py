@@ -134,3 +134,18 @@ def test_resume_with_another_path_count_uses_fresh_ids(tracker, tmp_path):
     assert np.array_equal(codes, np.concatenate(ecodes))
     assert np.array_equal(coll.view(np.float64), ecoll.view(np.float64))
     assert progress.count("-----Progress statistics for round ") == 3
+
+
+def test_failed_collection_write_keeps_the_previous_files(tmp_path):
+    """fptColl is written to fptColl.tmp and renamed over the old one: when that write fails (here
+    a directory of that name stands in the way), the run stops with an error and the files of the
+    earlier rounds stay as they were."""
+    run_rounds(tmp_path, 1)
+    coll0, codes0, _ = read_files(tmp_path)
+    os.mkdir(os.path.join(tmp_path, "fptColl.tmp"))
+    say("[gpu] frg_rounds 1 with fptColl.tmp blocked")
+    r = subprocess.run([EXE, "1", "--paths", str(N), "--dir", str(tmp_path)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 1 and "cannot write fptColl" in r.stderr, r.stdout + r.stderr
+    coll1, codes1, _ = read_files(tmp_path)
+    assert np.array_equal(coll0, coll1) and np.array_equal(codes0, codes1)
