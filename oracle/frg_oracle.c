@@ -11,8 +11,18 @@
  * FSYS, finite differences (JVAR, JPAR, dqdt), numpy.linalg.solve (LU solves), the RK6(5)
  * tableau's order conditions, Newton's quadratic convergence, the cognate invariant, scipy's
  * Lambert W and brute-force duplicate detection.
+ *
+ * Mutation testing: built with -DORACLE_MUTANT=k (tests/test_oracle_mutants.py does, into a
+ * temporary directory), the library plants the k-th of the deliberate mistakes marked MUT(k)
+ * below -- a swapped operand, a dropped term, a wrong sign or index -- so that the tests can show
+ * that one of the checks above catches each.  The Makefile builds no mutant.
  */
 #include "frg_oracle.h"
+
+#ifndef ORACLE_MUTANT
+#define ORACLE_MUTANT 0
+#endif
+#define MUT(k) (ORACLE_MUTANT == (k))
 
 #include <math.h>
 #include <stdlib.h>
@@ -84,8 +94,8 @@ void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t
     uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
     uint32_t k0 = key[0], k1 = key[1];
     for (int r = 0; r < 10; ++r) {
-        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
-        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint64_t p0 = (uint64_t)0xD2511F53u * (MUT(1) ? c2 : c0);
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * (MUT(1) ? c0 : c2);
         uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
         uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
         uint32_t n0 = hi1 ^ c1 ^ k0, n1 = lo1, n2 = hi0 ^ c3 ^ k1, n3 = lo0;
@@ -105,6 +115,7 @@ double oracle_uniform(uint64_t seed, uint64_t path_id, uint64_t draw) {
     uint32_t o[4];
     oracle_philox4x32_10(ctr, key, o);
     uint64_t w = (draw & 1) ? (((uint64_t)o[3] << 32) | o[2]) : (((uint64_t)o[1] << 32) | o[0]);
+    if (MUT(2)) w = (w << 32) | (w >> 32);
     return (double)((w >> 11) + 1) * 0x1.0p-53;
 }
 
@@ -247,6 +258,7 @@ static inline void shape_finish(eqshape *e) {
 }
 /* unconjugated variable */
 static inline ocplx d_unc(const eqshape *e, ocplx dK, ocplx dX, ocplx dY) {
+    if (MUT(3)) return cadd(cmul(e->Z0, dK), cmul(csub(cmul(e->Z0, dX), cmul(e->U, dY)), e->Wc));
     return csub(cmul(e->Z0, dK), cmul(csub(cmul(e->Z0, dX), cmul(e->U, dY)), e->Wc));
 }
 /* conjugated variable */
@@ -283,7 +295,7 @@ static void build_I(const ocplx *Z, const ocplx *q, int j, eqshape *e) {
     K = cadd(K, cmul(A, Cc));
     K = csub(K, cmul(H, Cc));
     K = cadd(K, cmul(H, Hc));
-    K = cadd(K, cmul(cmul(CA, CcAc), cmul(Q, Qc)));
+    if (!MUT(4)) K = cadd(K, cmul(cmul(CA, CcAc), cmul(Q, Qc)));
     e->K = K;
     shape_finish(e);
 }
@@ -436,8 +448,8 @@ void oracle_jpar(const ocplx *Z, const ocplx *q, ocplx *J) {
             ocplx *row = &J[(7 + j - 1) * 30];
             row[0] = cmul(cmul(e.U, e.Z0), e.Wc);
             row[j] = cneg(cmul(cmul(e.Z0, e.Z0), e.Wc));
-            row[8] = cmul(cmul(e.W, e.zu), e.Z0);
-            row[8 + j] = cneg(cmul(e.W, e.Z0));
+            row[MUT(5) ? 8 + j : 8] = cmul(cmul(e.W, e.zu), e.Z0);
+            row[MUT(5) ? 8 : 8 + j] = cneg(cmul(e.W, e.Z0));
         }
         {
             eqshape e; eq3aux a; build_III(Z, q, j, &e, &a);
@@ -466,7 +478,7 @@ void oracle_switch_patch(const ocplx *unew, const ocplx *Yold, ocplx *Ynew) {
     ocplx scalar = cdiv(csub(C(1, 0), cmul(ut, t)), den);
     ocplx tmp[ONY];
     for (int i = 0; i < ONV + 1; ++i) tmp[i] = cmul(scalar, Yold[i]);
-    tmp[ONV + 1] = t;
+    tmp[ONV + 1] = MUT(7) ? cmul(scalar, t) : t;
     memcpy(Ynew, tmp, sizeof(tmp));
 }
 
@@ -474,7 +486,7 @@ void oracle_switch_patch(const ocplx *unew, const ocplx *Yold, ocplx *Ynew) {
 void oracle_q(double t, const ocplx *spm, const ocplx *fpm, const ocplx *gamma, ocplx *out) {
     for (int i = 0; i < ONP; ++i) {
         out[i] = cdiv(cadd(cmul(cmul(*gamma, C(1. - t, 0)), spm[i]), cmul(C(t, 0), fpm[i])),
-                      cadd(cmul(*gamma, C(1. - t, 0)), C(t, 0)));
+                      MUT(6) ? cadd(cmul(*gamma, C(t, 0)), C(1. - t, 0)) : cadd(cmul(*gamma, C(1. - t, 0)), C(t, 0)));
     }
 }
 
@@ -528,14 +540,15 @@ void oracle_linear_solve(ocplx *mat, ocplx *b, ocplx *x) {
         double maxAbs = 0.0, val;
         int maxRow = i;
         for (int k = i; k < n; ++k) {
-            if ((val = a[k][i].re * a[k][i].re + a[k][i].im * a[k][i].im) > maxAbs) {
+            val = a[k][i].re * a[k][i].re + a[k][i].im * a[k][i].im;
+            if (MUT(16) ? val < maxAbs : val > maxAbs) {
                 maxAbs = val;
                 maxRow = k;
             }
         }
         if (maxRow != i) {
             ocplx *p = a[i]; a[i] = a[maxRow]; a[maxRow] = p;
-            ocplx tb = b[i]; b[i] = b[maxRow]; b[maxRow] = tb;
+            if (!MUT(8)) { ocplx tb = b[i]; b[i] = b[maxRow]; b[maxRow] = tb; }
         }
         for (int j = i + 1; j < n; ++j) {
             a[j][i] = cdiv(a[j][i], a[i][i]);
@@ -643,7 +656,7 @@ int oracle_newton(double tgoal, const ocplx *Vn, const ocplx *u, const ocplx *sp
         double tdiff = tgoal - YN[ONV + 1].re;
         for (int j = 0; j < ONV + 1; ++j) bvec[j] = crs(-1.0, Hv[j]);
         oracle_linear_solve_newton(M, bvec, tdiff, Yd);
-        for (int j = 0; j < ONY; ++j) YN[j] = cadd(YN[j], Yd[j]);
+        for (int j = 0; j < ONY; ++j) YN[j] = MUT(9) ? csub(YN[j], Yd[j]) : cadd(YN[j], Yd[j]);
         if (residual_type == 0) {
             oracle_H(YN, u, spm, fpm, gamma, Hv);
             *residual = vnorm(Hv, ONV + 1);
@@ -725,12 +738,13 @@ void oracle_rk65(double s, double tol, int tstep, const ocplx *Yc, const ocplx *
         rk_combo(NULL, 7, c, ks, LTE);
     }
     *err = vnorm(LTE, ONY);
-    if (*err > 10e-17) *scale = 0.8 * pow((tol / (*err)), 1. / 6.);
+    if (*err > 10e-17) *scale = 0.8 * pow((tol / (*err)), MUT(11) ? 1. / 5. : 1. / 6.);
     else *scale = 5.0;
     if (*scale > 5) *scale = 5;
     {
         double c[7] = {61. / 864. * s, 98415. / 321776. * s, 16807. / 146016. * s,
                        1375. / 7344. * s, 1375. / 5408. * s, -37. / 1120. * s, 1. / 10. * s};
+        if (MUT(10)) { const double t = c[3]; c[3] = c[4]; c[4] = t; }
         const ocplx *ks[7] = {k[0], k[2], k[3], k[4], k[5], k[6], k[7]};
         rk_combo(Yc, 7, c, ks, Yn);
     }
@@ -908,7 +922,7 @@ void oracle_generate_start(const otrk_opts *o, uint64_t seed, uint64_t path_id, 
                          cadd(cmul(cdiv(one, U[j]), csub(A, B)),
                               csub(csub(cmul(R, csub(H, Cv)), cmul(SUc, csub(F, B))), csub(H, F))));
         Q[j] = cmul(QUc, U[j]);
-        Qc[j] = cdiv(one, Q[j]);
+        Qc[j] = MUT(15) ? Q[j] : cdiv(one, Q[j]);
         ocplx S = cmul(SUc, U[j]);
         P[j] = cadd(B, cadd(cmul(S, csub(F, B)), cmul(U[j], csub(P[0], F))));
         Pc[j] = cadd(Bc, cadd(cmul(cdiv(one, S), csub(Fc, Bc)), cmul(cdiv(one, U[j]), csub(Pc[0], Fc))));
@@ -925,7 +939,7 @@ void oracle_cognate12(const ocplx *orig, ocplx P, ocplx *cogn) {
     cogn[0] = cadd(cmul(scalar, csub(A, B)), B);
     cogn[2] = cadd(cmul(scalar, csub(Cv, B)), B);
     cogn[3] = cadd(cmul(scalar, csub(Dp, B)), B);
-    cogn[4] = cadd(B, csub(P, F));
+    cogn[4] = cadd(B, MUT(12) ? csub(F, P) : csub(P, F));
     cogn[5] = cadd(cmul(scalar, csub(Gp, B)), B);
     cogn[6] = cadd(cmul(scalar, csub(H, B)), B);
     cogn[1] = B;
@@ -1100,7 +1114,8 @@ double oracle_lambertw(double z) {
     unsigned iter = 0;
     do {
         wprev = w;
-        w -= 2. * ((zexpz(w) - z) * zexpz_d(w)) / (2. * pow(zexpz_d(w), 2) - (zexpz(w) - z) * zexpz_dd(w));
+        const double dw = 2. * ((zexpz(w) - z) * zexpz_d(w)) / (2. * pow(zexpz_d(w), 2) - (zexpz(w) - z) * zexpz_dd(w));
+        w = MUT(13) ? w + dw : w - dw;
         iter++;
     } while ((fabs(w - wprev) > 1.E-30) && iter < 30);
     return w;
@@ -1124,7 +1139,7 @@ static int okey_cmp(const void *a, const void *b) {
     return (x->idx > y->idx) - (x->idx < y->idx);
 }
 static int same_root(const ocplx *a, const ocplx *b, double tol) {
-    for (int j = 0; j < ONV; ++j) {
+    for (int j = 0; j < (MUT(14) ? ONV - 1 : ONV); ++j) {
         if (fabs(a[j].re - b[j].re) > tol) return 0;
         if (fabs(a[j].im - b[j].im) > tol) return 0;
     }

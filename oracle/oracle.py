@@ -6,6 +6,7 @@ passed straight through: their layout is {double re, im}, the same as the oracle
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as ct
 import os
 
@@ -55,47 +56,69 @@ def lib(fast: bool = False) -> ct.CDLL:
         path = os.path.join(HERE, name)
         if not os.path.exists(path):
             raise RuntimeError(f"{path} is not built; run `make -C oracle`")
-        L = ct.CDLL(path)
-        P = ct.c_void_p
-        L.oracle_default_opts.argtypes = [ct.POINTER(Opts)]
-        L.oracle_philox4x32_10.argtypes = [P, P, P]
-        L.oracle_uniform.argtypes = [ct.c_uint64, ct.c_uint64, ct.c_uint64]
-        L.oracle_uniform.restype = ct.c_double
-        for fn in ("oracle_cdiv",):
-            getattr(L, fn).argtypes = [P, P, P]
-        L.oracle_csqrt.argtypes = [P, P]
-        L.oracle_refine_root.argtypes = [ct.POINTER(Opts), P, CCplx, P]
-        L.oracle_homotopy_explicit_ids.argtypes = [ct.POINTER(Opts), P, CCplx, P, P, ct.c_uint64, P, ct.c_int32,
-                                                   P, P, ct.c_int]
-        for fn in ("oracle_fsys", "oracle_fsys_quad", "oracle_jvar", "oracle_jpar"):
-            getattr(L, fn).argtypes = [P, P, P]
-        L.oracle_switch_patch.argtypes = [P, P, P]
-        for fn in ("oracle_q", "oracle_dqdt"):
-            getattr(L, fn).argtypes = [ct.c_double, P, P, P, P]
-        for fn in ("oracle_H", "oracle_JH"):
-            getattr(L, fn).argtypes = [P, P, P, P, P, P]
-        L.oracle_linear_solve.argtypes = [P, P, P]
-        L.oracle_linear_solve_newton.argtypes = [P, P, ct.c_double, P]
-        L.oracle_ivp.argtypes = [ct.c_int, P, P, P, P, P, P, P]
-        L.oracle_newton.argtypes = [ct.c_double, P, P, P, P, P, ct.c_int, ct.c_double, ct.c_int, P, P]
-        L.oracle_newton.restype = ct.c_int
-        L.oracle_rk65.argtypes = [ct.c_double, ct.c_double, ct.c_int, P, P, P, P, P, P, P, P, P, P]
-        L.oracle_track_path.argtypes = [ct.POINTER(Opts), P, P, P, P, ct.c_uint64, ct.c_uint64,
-                                        ct.c_uint64, P, P]
-        L.oracle_track_path.restype = ct.c_int
-        L.oracle_generate_start.argtypes = [ct.POINTER(Opts), ct.c_uint64, ct.c_uint64, P, P]
-        L.oracle_cognate12.argtypes = [P, CCplx, P]
-        L.oracle_cognate_solns.argtypes = [P, P, P]
-        L.oracle_homotopy.argtypes = [ct.POINTER(Opts), P, CCplx, ct.c_uint64, ct.c_uint64,
-                                      ct.c_int32, P, P, P, P, ct.c_int]
-        L.oracle_homotopy_explicit.argtypes = [ct.POINTER(Opts), P, CCplx, P, P, ct.c_uint64,
-                                               ct.c_uint64, ct.c_int32, P, P, P, P, ct.c_int]
-        L.oracle_lambertw.argtypes = [ct.c_double]
-        L.oracle_lambertw.restype = ct.c_double
-        L.oracle_sort_dedupe_merge.argtypes = [P, ct.c_int64, P, P, ct.c_int64, ct.c_double, P]
-        L.oracle_sort_dedupe_merge.restype = ct.c_int64
-        _LIBS[name] = L
+        _LIBS[name] = _configure(ct.CDLL(path))
     return _LIBS[name]
+
+
+def load(path: str) -> ct.CDLL:
+    """Another build of frg_oracle.c (a mutant of the mutation tests), configured like lib()."""
+    return _configure(ct.CDLL(path))
+
+
+@contextlib.contextmanager
+def using(L: ct.CDLL):
+    """Within the block, the functions of this module call L instead of liboracle.so."""
+    prev = _LIBS.get("liboracle.so")
+    _LIBS["liboracle.so"] = L
+    try:
+        yield
+    finally:
+        if prev is None:
+            _LIBS.pop("liboracle.so", None)
+        else:
+            _LIBS["liboracle.so"] = prev
+
+
+def _configure(L: ct.CDLL) -> ct.CDLL:
+    P = ct.c_void_p
+    L.oracle_default_opts.argtypes = [ct.POINTER(Opts)]
+    L.oracle_philox4x32_10.argtypes = [P, P, P]
+    L.oracle_uniform.argtypes = [ct.c_uint64, ct.c_uint64, ct.c_uint64]
+    L.oracle_uniform.restype = ct.c_double
+    for fn in ("oracle_cdiv",):
+        getattr(L, fn).argtypes = [P, P, P]
+    L.oracle_csqrt.argtypes = [P, P]
+    L.oracle_refine_root.argtypes = [ct.POINTER(Opts), P, CCplx, P]
+    L.oracle_homotopy_explicit_ids.argtypes = [ct.POINTER(Opts), P, CCplx, P, P, ct.c_uint64, P, ct.c_int32,
+                                               P, P, ct.c_int]
+    for fn in ("oracle_fsys", "oracle_fsys_quad", "oracle_jvar", "oracle_jpar"):
+        getattr(L, fn).argtypes = [P, P, P]
+    L.oracle_switch_patch.argtypes = [P, P, P]
+    for fn in ("oracle_q", "oracle_dqdt"):
+        getattr(L, fn).argtypes = [ct.c_double, P, P, P, P]
+    for fn in ("oracle_H", "oracle_JH"):
+        getattr(L, fn).argtypes = [P, P, P, P, P, P]
+    L.oracle_linear_solve.argtypes = [P, P, P]
+    L.oracle_linear_solve_newton.argtypes = [P, P, ct.c_double, P]
+    L.oracle_ivp.argtypes = [ct.c_int, P, P, P, P, P, P, P]
+    L.oracle_newton.argtypes = [ct.c_double, P, P, P, P, P, ct.c_int, ct.c_double, ct.c_int, P, P]
+    L.oracle_newton.restype = ct.c_int
+    L.oracle_rk65.argtypes = [ct.c_double, ct.c_double, ct.c_int, P, P, P, P, P, P, P, P, P, P]
+    L.oracle_track_path.argtypes = [ct.POINTER(Opts), P, P, P, P, ct.c_uint64, ct.c_uint64,
+                                    ct.c_uint64, P, P]
+    L.oracle_track_path.restype = ct.c_int
+    L.oracle_generate_start.argtypes = [ct.POINTER(Opts), ct.c_uint64, ct.c_uint64, P, P]
+    L.oracle_cognate12.argtypes = [P, CCplx, P]
+    L.oracle_cognate_solns.argtypes = [P, P, P]
+    L.oracle_homotopy.argtypes = [ct.POINTER(Opts), P, CCplx, ct.c_uint64, ct.c_uint64,
+                                  ct.c_int32, P, P, P, P, ct.c_int]
+    L.oracle_homotopy_explicit.argtypes = [ct.POINTER(Opts), P, CCplx, P, P, ct.c_uint64,
+                                           ct.c_uint64, ct.c_int32, P, P, P, P, ct.c_int]
+    L.oracle_lambertw.argtypes = [ct.c_double]
+    L.oracle_lambertw.restype = ct.c_double
+    L.oracle_sort_dedupe_merge.argtypes = [P, ct.c_int64, P, P, ct.c_int64, ct.c_double, P]
+    L.oracle_sort_dedupe_merge.restype = ct.c_int64
+    return L
 
 
 def default_opts(**kw) -> Opts:

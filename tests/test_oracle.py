@@ -235,6 +235,28 @@ def test_rk65_orders():
     assert 5.3 < order_est < 6.8, order_est
 
 
+def test_step_factor_targets_the_tolerance():
+    """The step-size factor of an RK attempt (rungekutta.cu:107-112) aims the next attempt at the
+    tolerance: a step of s * scale / 0.8 (the factor without its safety margin) has an error
+    estimate within 25 % of tol, because the estimate is O(s^6).  An exponent that does not
+    match the estimate's order misses by a factor of 4 or more.  Also the reference's limits:
+    at most 5, and 5 for an estimate at or below 1e-16."""
+    Y, u, qs = _point_on_path()
+    fpm, g = tasks.REF_FPM, tasks.REF_GAMMA
+    V = O.ivp(0, Y, np.ones(23, complex), u, qs, fpm, g)
+    for s0 in (0.05, 0.1):
+        _, _, err0, _ = O.rk65(s0, 1.0, 0, Y, V, u, qs, fpm, g)
+        for ratio in (1e-3, 1e-5):
+            tol = err0 * ratio
+            _, _, _, scale = O.rk65(s0, tol, 0, Y, V, u, qs, fpm, g)
+            _, _, err1, _ = O.rk65(s0 * scale / 0.8, tol, 0, Y, V, u, qs, fpm, g)
+            assert 0.8 < err1 / tol < 1.25, (s0, ratio, err1 / tol)
+        _, _, _, scale = O.rk65(s0, err0 * 1e9, 0, Y, V, u, qs, fpm, g)
+        assert scale == 5.0
+    _, _, err, scale = O.rk65(1e-9, 1.0, 0, Y, V, u, qs, fpm, g)
+    assert err <= 1e-16 and scale == 5.0
+
+
 def test_newton_converges_quadratically():
     Y, u, qs = _point_on_path()
     fpm, g = tasks.REF_FPM, tasks.REF_GAMMA
@@ -361,6 +383,25 @@ def test_sort_dedupe_merge_vs_brute_force():
     assert np.array_equal(err2, _brute_dedupe(coll, fpt, err, 1e-6))
     assert np.all(np.diff(out[:, 0].real) >= 0)
     assert len(out) == len(coll) + int((err2 == 0).sum())
+
+
+def test_dedupe_compares_every_real():
+    """Two roots are duplicates only when all 42 reals agree within tol (frgsupport.cu:7, 69-171):
+    a copy of a collected root that differs by 10 tol in any single real, whichever it is, is
+    new; a copy within tol / 2 in every real is a duplicate."""
+    rng = np.random.default_rng(17)
+    tol = 1e-6
+    coll = rng.normal(size=(42, 21)) + 1j * rng.normal(size=(42, 21))
+    coll = coll[np.argsort(coll[:, 0].real)]
+    far = coll.copy().view(np.float64)          # root r differs in real number r only
+    far[np.arange(42), np.arange(42)] += 10 * tol
+    near = coll + 0.5 * tol * (rng.uniform(-1, 1, (42, 21)) + 1j * rng.uniform(-1, 1, (42, 21)))
+    fpt = np.concatenate([far.view(np.complex128), near])
+    err = np.zeros(84, np.int32)
+    out, err2 = O.sort_dedupe_merge(coll, fpt, err, tol)
+    assert np.array_equal(err2, _brute_dedupe(coll, fpt, err, tol))
+    assert np.all(err2[:42] == 0) and np.all(err2[42:] == 6)
+    assert len(out) == 84
 
 
 def test_reference_fixtures_consistent():

@@ -1,0 +1,69 @@
+"""Mutation tests of the oracle's pins: each case builds oracle/frg_oracle.c with one deliberate,
+plausible mistake planted (-DORACLE_MUTANT=k: a swapped operand, a dropped term, a wrong sign or
+index; the sites are marked MUT(k) in the source) into a temporary directory, and shows that
+the named check of tests/test_oracle.py fails on that build while it passes on the real one.
+The mistakes live only in these temporary CPU builds; nothing here touches a GPU."""
+import concurrent.futures as cf
+import functools
+import os
+import re
+import subprocess
+
+import pytest
+
+from oracle import oracle as O
+from tests import test_oracle as T
+
+SRC = os.path.join(os.path.dirname(O.__file__), "frg_oracle.c")
+CFLAGS = ["-std=gnu11", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-O2"]
+
+# (k, the mistake, the check that must catch it)
+MUTANTS = [
+    (1, "Philox: multipliers applied to swapped counter words", T.test_philox_known_answers),
+    (2, "uniform draw: 32-bit halves of the 64-bit word swapped", T.test_uniform_draws),
+    (3, "JVAR: wrong sign of the product-rule term of an unconjugated variable",
+     T.test_jvar_jpar_finite_differences),
+    (4, "JVAR: last term of K_j of equation I dropped", T.test_jvar_jpar_finite_differences),
+    (5, "JPAR: columns Pc0 and Pc_j of equation II transposed", T.test_jvar_jpar_finite_differences),
+    (6, "q(t): 1 - t and t transposed in the denominator", T.test_q_and_dqdt),
+    (7, "SwitchPatch: t rescaled with the other coordinates", T.test_switch_patch_with_reference_patches),
+    (8, "LinearSolve: rows exchanged without their right-hand sides", T.test_linear_solve_vs_lapack),
+    (9, "Newton: update applied with the wrong sign", T.test_newton_converges_quadratically),
+    (10, "RK6(5)8M: two weights of the propagated solution exchanged", T.test_rk65_orders),
+    (11, "step-size factor with the exponent of a fifth-order estimate",
+     T.test_step_factor_targets_the_tolerance),
+    (12, "Cognate12: P - F taken the wrong way round", T.test_cognate_is_a_root),
+    (13, "Lambert W: Halley step added instead of subtracted", T.test_lambertw),
+    (14, "duplicate test: last variable not compared", T.test_dedupe_compares_every_real),
+    (15, "GenerateStart: Qc set to Q instead of 1/Q", functools.partial(T.test_start_points_are_roots, 0)),
+    (16, "LinearSolve: no row exchange (pivot search inverted)", T.test_linear_solve_vs_lapack),
+]
+
+
+@pytest.fixture(scope="module")
+def mutant_libs(tmp_path_factory):
+    d = tmp_path_factory.mktemp("oracle_mutants")
+
+    def build(k):
+        out = str(d / f"liboracle_mutant{k}.so")
+        r = subprocess.run(["gcc", *CFLAGS, f"-DORACLE_MUTANT={k}", "-o", out, SRC, "-lm"], capture_output=True,
+                           text=True)
+        assert r.returncode == 0, r.stderr
+        return k, out
+
+    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        return dict(ex.map(build, [k for k, _, _ in MUTANTS]))
+
+
+def test_every_site_in_the_source_has_a_case():
+    src = open(SRC).read()
+    sites = {int(x) for x in re.findall(r"MUT\((\d+)\)", src)}
+    assert sites == {k for k, _, _ in MUTANTS}
+
+
+@pytest.mark.parametrize("k,what,check", MUTANTS, ids=[f"m{k}" for k, _, _ in MUTANTS])
+def test_mutant_is_caught(mutant_libs, k, what, check):
+    check()  # the real oracle passes
+    with O.using(O.load(mutant_libs[k])):
+        with pytest.raises(AssertionError):
+            check()
