@@ -1,18 +1,14 @@
-"""Pins the numpy oracle of the step-size model (oracle/mlp_oracle.py) by finite differences,
-and checks its parameter layout against the C ABI's constants."""
+"""Pins the numpy oracle of the step-size model (oracle/mlp_oracle.py): its gradient by finite
+differences, its forward pass by a scalar loop over the documented parameter layout, forward and
+loss against each other, and the layout against the C ABI's constants.  The check_* functions
+take the oracle module as an argument so that the mutation tests can run them on altered copies."""
 import numpy as np
 
 from finite_root_generation_amd import capi
 from oracle import mlp_oracle as M
 
 
-def test_layout_matches_abi():
-    assert M.NW == capi.STEP_MODEL_WEIGHTS == 4801
-    assert M.SIZE == capi.STEP_MODEL_SIZE == 4819
-    assert M.NIN + 1 == capi.STEP_RECORD
-
-
-def test_gradient_finite_differences():
+def check_gradient_finite_differences(M=M):
     rng = np.random.default_rng(0)
     rec = np.concatenate([rng.normal(size=(40, 8)), rng.normal(size=(40, 1))], axis=1)
     m = M.random_model(rng, rec)
@@ -26,7 +22,7 @@ def test_gradient_finite_differences():
         assert abs(fd - g[p]) <= 1e-7 + 1e-5 * abs(fd), p
 
 
-def test_forward_consistent_with_loss():
+def check_forward_consistent_with_loss(M=M):
     rng = np.random.default_rng(1)
     rec = np.concatenate([rng.normal(size=(30, 8)), rng.normal(size=(30, 1))], axis=1)
     m = M.random_model(rng, rec)
@@ -34,3 +30,41 @@ def test_forward_consistent_with_loss():
     t = rec[:, 8]
     loss = 0.5 * np.mean(((y - t) / m[M.YSD]) ** 2)
     assert abs(loss - M.loss_and_grad(m, rec)[0]) < 1e-12
+
+
+def check_forward_matches_scalar_definition(M=M):
+    """The model as the parameter block defines it, one scalar at a time: hidden unit j of the
+    first layer is tanh(sum_i W1[j][i] x_i + b1[j]) with W1 stored row-major at offset 0 (8 per
+    row), the second layer likewise with W2[j][i] at 576 + 64 j + i, and the output
+    sum_j w3[j] h2[j] + b3, with x and the output standardised by xmu, xsd, ymu, ysd."""
+    rng = np.random.default_rng(2)
+    rec = np.concatenate([rng.normal(size=(20, 8)), rng.normal(size=(20, 1))], axis=1)
+    m = M.random_model(rng, rec)
+    nin, nh = 8, 64
+    w1, b1, w2, b2, w3 = 0, nh * nin, nh * nin + nh, nh * nin + nh + nh * nh, nh * nin + 2 * nh + nh * nh
+    b3, xmu, xsd, ymu, ysd = w3 + nh, w3 + nh + 1, w3 + nh + 1 + nin, w3 + nh + 1 + 2 * nin, w3 + nh + 2 + 2 * nin
+    for f in rec[:4, :8]:
+        x = [(f[i] - m[xmu + i]) / m[xsd + i] for i in range(nin)]
+        h1 = [np.tanh(sum(m[w1 + nin * j + i] * x[i] for i in range(nin)) + m[b1 + j]) for j in range(nh)]
+        h2 = [np.tanh(sum(m[w2 + nh * j + i] * h1[i] for i in range(nh)) + m[b2 + j]) for j in range(nh)]
+        y = sum(m[w3 + j] * h2[j] for j in range(nh)) + m[b3]
+        ref = y * m[ysd] + m[ymu]
+        assert abs(M.forward(m, f[None, :])[0] - ref) <= 1e-12 * (1 + abs(ref))
+
+
+def test_layout_matches_abi():
+    assert M.NW == capi.STEP_MODEL_WEIGHTS == 4801
+    assert M.SIZE == capi.STEP_MODEL_SIZE == 4819
+    assert M.NIN + 1 == capi.STEP_RECORD
+
+
+def test_gradient_finite_differences():
+    check_gradient_finite_differences()
+
+
+def test_forward_consistent_with_loss():
+    check_forward_consistent_with_loss()
+
+
+def test_forward_matches_scalar_definition():
+    check_forward_matches_scalar_definition()

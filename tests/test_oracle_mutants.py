@@ -2,16 +2,20 @@
 plausible mistake planted (-DORACLE_MUTANT=k: a swapped operand, a dropped term, a wrong sign or
 index; the sites are marked MUT(k) in the source) into a temporary directory, and shows that
 the named check of tests/test_oracle.py fails on that build while it passes on the real one.
-The mistakes live only in these temporary CPU builds; nothing here touches a GPU."""
+The numpy oracle of the step-size model (oracle/mlp_oracle.py) gets the same treatment from an
+altered copy of its source executed as a separate module.  The mistakes live only in these
+temporary CPU copies; nothing here touches a GPU."""
 import concurrent.futures as cf
 import functools
 import os
 import re
 import subprocess
+import types
 
 import pytest
 
 from oracle import oracle as O
+from tests import test_mlp_oracle as TM
 from tests import test_oracle as T
 
 SRC = os.path.join(os.path.dirname(O.__file__), "frg_oracle.c")
@@ -67,3 +71,36 @@ def test_mutant_is_caught(mutant_libs, k, what, check):
     with O.using(O.load(mutant_libs[k])):
         with pytest.raises(AssertionError):
             check()
+
+
+MLP_SRC = os.path.join(os.path.dirname(O.__file__), "mlp_oracle.py")
+
+# (the mistake, source text, altered text, the check that must catch it)
+MLP_MUTANTS = [
+    ("second layer with W2 transposed (forward and loss alike)", "h1 @ w2.T", "h1 @ w2",
+     TM.check_forward_matches_scalar_definition),
+    ("tanh derivative of the second layer dropped from the gradient", "np.outer(dy, w3) * (1 - h2 * h2)",
+     "np.outer(dy, w3)", TM.check_gradient_finite_differences),
+    ("gradient of W1 stored transposed", "(d1.T @ x).ravel()", "(x.T @ d1).ravel()",
+     TM.check_gradient_finite_differences),
+    ("features multiplied by their deviation in the forward pass",
+     "x = (np.asarray(feats, np.float64) - xmu) / xsd", "x = (np.asarray(feats, np.float64) - xmu) * xsd",
+     TM.check_forward_consistent_with_loss),
+    ("output mean left out of the forward pass", "return y * ysd + ymu", "return y * ysd",
+     TM.check_forward_consistent_with_loss),
+]
+
+
+def mlp_mutant(old, new):
+    src = open(MLP_SRC).read()
+    assert old in src, old
+    mod = types.ModuleType("mlp_oracle_mutant")
+    exec(compile(src.replace(old, new), MLP_SRC + " (mutant)", "exec"), mod.__dict__)
+    return mod
+
+
+@pytest.mark.parametrize("what,old,new,check", MLP_MUTANTS, ids=[f"mlp{i}" for i in range(len(MLP_MUTANTS))])
+def test_mlp_mutant_is_caught(what, old, new, check):
+    check()  # the real oracle passes
+    with pytest.raises(AssertionError):
+        check(mlp_mutant(old, new))
