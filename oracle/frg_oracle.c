@@ -46,7 +46,7 @@ static inline ocplx cmul(ocplx a, ocplx b) {
 static inline ocplx cdiv(ocplx a, ocplx b) {
     if (fabs(b.re) >= fabs(b.im)) {
         double r = b.im / b.re, d = b.re + b.im * r;
-        return C((a.re + a.im * r) / d, (a.im - a.re * r) / d);
+        return C((a.re + a.im * r) / d, (MUT(17) ? a.im + a.re * r : a.im - a.re * r) / d);
     } else {
         double r = b.re / b.im, d = b.im + b.re * r;
         return C((a.re * r + a.im) / d, (a.im * r - a.re) / d);
@@ -66,7 +66,7 @@ static inline ocplx csqrt_ref(ocplx x) {
     double r = cabs_(x);
     double cosA = x.re / r;
     ocplx out = C(sqrt(r * (cosA + 1.0) / 2.0), sqrt(r * (1.0 - cosA) / 2.0));
-    if (signbit(x.im)) out.im *= -1.0;
+    if (MUT(18) ? !signbit(x.im) : signbit(x.im)) out.im *= -1.0;
     return out;
 }
 
@@ -588,7 +588,7 @@ void oracle_linear_solve_newton(ocplx *mat, ocplx *b, double tdiff, ocplx *x) {
             b[j] = csub(b[j], cmul(b[i], a[j][i]));
         }
     }
-    x[n - 1] = C(tdiff, 0.);
+    x[n - 1] = C(MUT(20) ? -tdiff : tdiff, 0.);
     for (int i = n - 2; i > -1; --i) {
         ocplx sum = C(0, 0);
         for (int k = i + 1; k < n; ++k) sum = cfma(a[i][k], x[k], sum);
@@ -628,7 +628,7 @@ void oracle_ivp(int tstep, const ocplx *Yc, const ocplx *Vc, const ocplx *u, con
     ocplx sc;
     if (!tstep) {
         double theta = atan2(x[ONV + 1].im, x[ONV + 1].re);
-        sc = cmul(C(cos(theta), -sin(theta)), C(1.0 / vnorm(x, ONY), 0));
+        sc = cmul(C(cos(theta), -sin(theta)), C(MUT(19) ? 1.0 : 1.0 / vnorm(x, ONY), 0));
     } else {
         sc = cdiv(C(1.0, 0), x[ONV + 1]);
     }

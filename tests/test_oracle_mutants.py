@@ -19,6 +19,7 @@ from tests import test_mlp_oracle as TM
 from tests import test_oracle as T
 
 SRC = os.path.join(os.path.dirname(O.__file__), "frg_oracle.c")
+INC = os.path.join(os.path.dirname(O.__file__), "fsys_expanded.inc")
 CFLAGS = ["-std=gnu11", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-fno-fast-math", "-O2"]
 
 # (k, the mistake, the check that must catch it)
@@ -41,6 +42,13 @@ MUTANTS = [
     (14, "duplicate test: last variable not compared", T.test_dedupe_compares_every_real),
     (15, "GenerateStart: Qc set to Q instead of 1/Q", functools.partial(T.test_start_points_are_roots, 0)),
     (16, "LinearSolve: no row exchange (pivot search inverted)", T.test_linear_solve_vs_lapack),
+    (17, "complex division: wrong sign in the imaginary part (Smith's first branch)",
+     T.test_complex_division_and_sqrt),
+    (18, "complex sqrt: sign of the imaginary part taken from the wrong half plane",
+     T.test_complex_division_and_sqrt),
+    (19, "IVP: arclength tangent not normalised", T.test_ivp_tangent_properties),
+    (20, "LinearSolveNewton: x_t pinned to -dt", T.test_linear_solve_newton_ignores_last_row_and_pins_t),
+    (21, "FSYS II_j: Fc read as F in the fourth term", T.test_fsys_matches_independent_form),
 ]
 
 
@@ -60,7 +68,7 @@ def mutant_libs(tmp_path_factory):
 
 
 def test_every_site_in_the_source_has_a_case():
-    src = open(SRC).read()
+    src = open(SRC).read() + open(INC).read()
     sites = {int(x) for x in re.findall(r"MUT\((\d+)\)", src)}
     assert sites == {k for k, _, _ in MUTANTS}
 
