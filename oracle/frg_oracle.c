@@ -794,18 +794,18 @@ int oracle_track_path(const otrk_opts *o, const ocplx *spt, const ocplx *spm, co
                                 &scale);
                     S.rk_attempts++;
                     S.ivp_calls += 8;
-                    if (ministep++ == o->max_ministeps) { code = 2; goto done; }
+                    if ((MUT(25) ? ++ministep : ministep++) == o->max_ministeps) { code = 2; goto done; }
                     if (err <= o->rk_tol) break;
                     if (s <= o->smin) { code = 3; goto done; }
                     S.rejections++;
                     s *= scale;
                     if (s < o->smin) s = o->smin * 0.9;
-                    streak = 0;
+                    if (!MUT(23)) streak = 0;
                 }
                 if (Yn[ONV + 1].re <= 1. - o->nudge_trigger) break;
                 if (fabs(1. - Yn[ONV + 1].re) < o->sharp_trigger) break;
                 tstepQ = 1;
-                s = 1 - Yc[ONV + 1].re;
+                s = 1 - (MUT(26) ? Yn : Yc)[ONV + 1].re;
             }
             if (fabs(1. - Yn[ONV + 1].re) < o->sharp_trigger) break;
             int i = 0;
@@ -816,9 +816,11 @@ int oracle_track_path(const otrk_opts *o, const ocplx *spt, const ocplx *spm, co
                 S.newton_calls++;
                 if (ministep++ == o->max_ministeps) { code = 2; goto done; }
                 if (residual < o->newt_tol) break;
-                draw_patch(&g, o, u);
+                if (!MUT(24)) {
+                    draw_patch(&g, o, u);
+                    oracle_switch_patch(u, Yn, Yn);
+                }
                 S.redraws++;
-                oracle_switch_patch(u, Yn, Yn);
                 i++;
             }
             if (residual < o->newt_tol) break;
@@ -831,7 +833,7 @@ int oracle_track_path(const otrk_opts *o, const ocplx *spt, const ocplx *spm, co
         memcpy(Yc, YN, sizeof(YN));
         memcpy(Vc, Vn, sizeof(Vn));
         streak++;
-        if (streak < o->caution_steps && scale > 1.) scale = 1.;
+        if (!MUT(22) && streak < o->caution_steps && scale > 1.) scale = 1.;
         s *= scale;
         if (s > o->smax) s = o->smax;
         else if (s < o->smin) s = o->smin * 0.9;

@@ -68,9 +68,15 @@ def mutant_libs(tmp_path_factory):
 
 
 def test_every_site_in_the_source_has_a_case():
+    """Every MUT(k) site is exercised: here on the CPU, or (TrackPath's bookkeeping) by the
+    comparison with the GPU tracker in tests/test_gpu_oracle_mutants.py."""
+    from tests.test_gpu_oracle_mutants import TRACKPATH_MUTANTS
     src = open(SRC).read() + open(INC).read()
     sites = {int(x) for x in re.findall(r"MUT\((\d+)\)", src)}
-    assert sites == {k for k, _, _ in MUTANTS}
+    cpu, gpu = {k for k, _, _ in MUTANTS}, {k for k, _, _ in TRACKPATH_MUTANTS}
+    assert not cpu & gpu and sites == cpu | gpu
+    makefile = open(os.path.join(os.path.dirname(SRC), "Makefile")).read()
+    assert re.search(r"TRACKPATH_MUTANTS = (.*)", makefile).group(1).split() == [str(k) for k in sorted(gpu)]
 
 
 @pytest.mark.parametrize("k,what,check", MUTANTS, ids=[f"m{k}" for k, _, _ in MUTANTS])
